@@ -1,0 +1,235 @@
+"""Benchmark of the wavefront path-tracing hot path on one MI355X.
+
+A step is one wave: one camera path per pixel of the workload's target,
+traced to the depth cap and folded into the film (Renderer::accumulate,
+render/mod.rs:245-259). The default workload is the material chart —
+1280x720, depth 12, thin lens 0.02, seeded procedural sky — and the default
+run times 64 steps, one full 64-spp render. The JSON line reports camera
+paths per second over the timed region, the dominant kernel's algorithmic
+bandwidth against the HBM roofline (HIP events on the launch stream), and
+the CPU oracle timed on a bounded slice of the same workload.
+
+    python bench.py [--gpus 1] [--steps 64] [--warmup 4] [--workload chart]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mpaths/s + p50/p99 first-frame latency, served vs plain, at relMSE 1e-3"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit of work, in the reference's record layout
+# (crates/cenote/src/wavefront.rs:211-356, shaders/pathstate.slang): a build
+# that compresses state is credited in time, not bytes.
+BYTES = {
+    "camera_path": 120,   # raygen 56 + zero-fill 16 + accumulate 48
+    "camera_path_aov": 144,
+    "ray": 60,            # intersect: state read 40 + hit/queue write 20
+    "hit": 200 + 396,     # shade_surface: state 76 + write-back 56 + shadow 68; scene records 396
+    "miss": 72 + 68,      # shade_miss: state 40 + radiance RMW 32; env texels + pdf 68
+    "shadow": 96,         # trace_shadow: record 64 + radiance RMW 32
+}
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(counts, traversal, scene_info, pixels, waves, aov):
+    """Per-kernel algorithmic bytes over `waves` waves."""
+    node, tri = scene_info.bvh_node_bytes, scene_info.bvh_triangle_bytes
+    rays = int(counts["rays"].sum())
+    hits = int(counts["hits"].sum())
+    misses = int(counts["misses"].sum())
+    shadows = int(counts["shadows"].sum())
+    return {
+        "raygen": pixels * waves * 72,
+        "accumulate": pixels * waves * (BYTES["camera_path_aov"] if aov else 48),
+        "intersect": BYTES["ray"] * rays + node * traversal["ray_nodes"] + tri * traversal["ray_triangles"],
+        "shade_surface": BYTES["hit"] * hits,
+        "shade_miss": BYTES["miss"] * misses,
+        "trace_shadow": BYTES["shadow"] * shadows + node * traversal["shadow_nodes"]
+        + tri * traversal["shadow_triangles"],
+    }, dict(rays=rays, hits=hits, misses=misses, shadows=shadows)
+
+
+def cpu_baseline(data, width, height, max_bounces, budget_s=12.0):
+    """The CPU oracle (oracle/, a C++ restatement of the reference estimator)
+    on the host's cores, over a bounded band of rows of the same workload."""
+    from oracle import oracle as orc
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    scene = orc.OracleScene(data)
+    # Probe a band of rows, then trace whole frames at successive sample
+    # indices (or a band, if one frame is over budget) until ~budget_s.
+    t0 = time.perf_counter()
+    scene.trace_rows(width, height, 0, 16, 0, max_bounces=max_bounces, threads=threads)
+    frame_s = (time.perf_counter() - t0) * height / 16
+    if frame_s > budget_s:
+        rows = max(16, int(height * budget_s / frame_s))
+        row0 = (height - rows) // 2
+        t0 = time.perf_counter()
+        scene.trace_rows(width, height, row0, row0 + rows, 0, max_bounces=max_bounces, threads=threads)
+        elapsed = time.perf_counter() - t0
+        paths = rows * width
+        sample = f"rows {row0}..{row0 + rows} of one {width}x{height} wave (sample 0)"
+    else:
+        waves = max(1, int(budget_s / max(frame_s, 1e-3)))
+        t0 = time.perf_counter()
+        for s in range(waves):
+            scene.trace(width, height, s, max_bounces=max_bounces, threads=threads)
+        elapsed = time.perf_counter() - t0
+        paths = waves * width * height
+        sample = f"{waves} full {width}x{height} waves (samples 0..{waves - 1})"
+    return {
+        "value": round(paths / elapsed / 1e6, 4),
+        "unit": "Mpaths/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample}, depth {max_bounces}: {paths} camera paths in {elapsed:.2f} s",
+    }
+
+
+def load_traffic(kernel: str, workload: str):
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    try:
+        table = json.loads(path.read_text())
+        entry = table.get(workload, {}).get(kernel)
+        return None if entry is None else entry.get("bytes_per_launch")
+    except (ValueError, AttributeError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="chart", choices=["cornell", "chart", "heavy"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by tile or "
+              "view across GPUs, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import cenote_amd as cn
+    from cenote_amd import workloads
+
+    cfg = workloads.CONFIGS[args.workload]
+    width, height, depth, aov = cfg["width"], cfg["height"], cfg["max_bounces"], cfg["aov"]
+    t0 = time.perf_counter()
+    data = workloads.build(args.workload)
+    gen_s = time.perf_counter() - t0
+    log(f"workload {args.workload}: {data.triangle_count} triangles, generated in {gen_s:.1f} s")
+
+    ctx = cn.Context(0)
+    name, cus = ctx.describe()
+    scene = cn.Scene(ctx, data)
+    info = scene.info()
+    log(f"device {name} ({cus} CUs); scene prep {info.prep_seconds:.2f} s (BVH {info.bvh_build_seconds:.2f} s, "
+        f"{info.bvh_node_count} nodes, depth {info.bvh_max_depth})")
+    renderer = cn.Renderer(ctx, max_bounces=depth)
+    film = cn.Film(ctx, width, height)
+    pixels = width * height
+    steps = args.steps
+
+    # Counting pass: the same waves the timed region traces, with traversal
+    # counters on, for the algorithmic-bytes model.
+    renderer.set_counters(True)
+    counts_total = None
+    for s in range(steps):
+        renderer.accumulate(scene, film)
+        c = renderer.wave_counts()
+        counts_total = c if counts_total is None else {k: counts_total[k] + c[k] for k in c}
+    traversal = renderer.traversal_counts()
+    renderer.set_counters(False)
+
+    # Warmup, then the timed region: the same sample indices 0..steps-1.
+    film.reset()
+    renderer.accumulate(scene, film, count=args.warmup)
+    film.reset()
+    renderer.set_profiling(True)
+    renderer.synchronize()
+    t0 = time.perf_counter()
+    renderer.accumulate(scene, film, count=steps, synchronize=False)
+    renderer.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = renderer.kernel_times()
+    renderer.set_profiling(False)
+
+    value = pixels * steps / elapsed / 1e6
+    per_kernel_bytes, totals = algorithmic_bytes(counts_total, traversal, info, pixels, steps, aov)
+    dominant = max(times, key=lambda k: times[k][0])
+    dom_ms, dom_launches = times[dominant]
+    dom_bytes = per_kernel_bytes.get(dominant, 0)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    grays = (totals["rays"] + totals["shadows"]) / elapsed / 1e9
+    kernel_ms = {k: round(v[0] / steps, 4) for k, v in times.items()}
+    kernel_gbs = {k: round(per_kernel_bytes[k] / (times[k][0] * 1e-3) / 1e9, 1)
+                  for k in times if times[k][0] > 0 and k in per_kernel_bytes}
+    log(f"timed {steps} waves in {elapsed:.3f} s: {value:.1f} Mpaths/s, {grays:.3f} Grays/s")
+    log(f"per-wave kernel ms: {kernel_ms}")
+    log(f"per-kernel algorithmic GB/s: {kernel_gbs}")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload} {width}x{height}, {steps} spp timed, depth {depth}"
+                        + (", thin lens 0.02" if args.workload == "chart" else "")
+                        + (", AOVs" if aov else ""),
+            "triangles": data.triangle_count,
+            "pixels": pixels,
+            "path_pool": cn.DEFAULT_CAPACITY,
+            "form": "plain (one progressive frame per step)",
+        },
+        "grays_per_s": round(grays, 4),
+        "kernel_ms_per_step": kernel_ms,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(dominant, args.workload),
+            "algorithmic_bytes_per_launch": int(dom_bytes / max(1, dom_launches)),
+            "avg_launch_ms": round(dom_ms / max(1, dom_launches), 5),
+        },
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(data, width, height, depth)
+        log(f"cpu baseline: {result['cpu_baseline']}")
+    film.close()
+    renderer.close()
+    scene.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

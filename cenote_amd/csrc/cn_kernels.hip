@@ -1,0 +1,896 @@
+// cn_kernels.hip — the wavefront stage kernels for gfx950.
+//
+// One wave traces one sample of every pixel of a target
+// (crates/cenote/src/wavefront.rs:1-28): raygen, then per bounce intersect ->
+// shade_miss | shade_surface -> trace_shadow, then the film's accumulate.
+// Stages talk through device queues whose counts never cross back to the
+// host: every kernel after raygen is launched with a fixed, chip-filling
+// grid and walks its queue with a grid-stride loop, reading the count the
+// previous kernel left (HIP's stand-in for the reference's indirect
+// dispatch, wavefront.rs:799-828). Queue pushes are aggregated per wave64 —
+// one ballot, one popcount, one atomic per wave — instead of the one global
+// atomic per push of shaders/pathstate.slang:277-283.
+//
+// Radiance and AOV writes are pixel-owned read-modify-writes: a wave holds
+// one path per pixel, and stream order serialises the stages, so the adds
+// land in the same per-pixel order on every run (pathstate.slang:165-171).
+
+#include <hip/hip_runtime.h>
+
+#include "cn_launch.h"
+#include "cn_math.h"
+#include "cn_records.h"
+#include "cn_shading.h"
+
+namespace cn {
+
+constexpr uint32_t BLOCK = 256;
+constexpr uint32_t ROULETTE_START = 3;   // shade_surface.slang:53
+constexpr uint32_t TRANSPARENT_CAP = 8;  // intersect.slang:23
+
+// --- wave64 queue aggregation ------------------------------------------------
+
+// Claim a slot in `counter` for every lane whose `active` is set, with one
+// atomic per wave. Must be reached by every lane of the wave.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t* counter, bool active) {
+    const uint64_t mask = __ballot(active);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + below;
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* target, uint32_t value) {
+    uint32_t v = value;
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset);
+    if (__lane_id() == 0 && v != 0) atomicAdd(target, (unsigned long long)v);
+}
+
+__device__ __forceinline__ void add_pixel(float4* target, uint32_t pixel, v3 c, float a) {
+    float4 v = target[pixel];
+    v.x += c.x;
+    v.y += c.y;
+    v.z += c.z;
+    v.w += a;
+    target[pixel] = v;
+}
+
+// --- traversal ---------------------------------------------------------------
+
+// The traversal stack: the first STACK_LDS entries of each lane live in LDS
+// (entry k of thread t at [k * BLOCK + t], bank-conflict free), the rest in
+// a per-lane spill area in HBM laid out [entry][lane] so spills coalesce.
+struct Stack {
+    uint32_t* lds;
+    uint32_t* spill;
+    uint32_t lanes;
+    uint32_t lane;
+    uint32_t sp;
+    // The builder rejects trees that could need more than STACK_MAX entries;
+    // a push past it is dropped all the same, so every pop stays in bounds.
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (sp >= STACK_MAX) return;
+        if (sp < STACK_LDS) lds[sp * BLOCK + threadIdx.x] = v;
+        else spill[(sp - STACK_LDS) * lanes + lane] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        return sp < STACK_LDS ? lds[sp * BLOCK + threadIdx.x] : spill[(sp - STACK_LDS) * lanes + lane];
+    }
+};
+
+__device__ __forceinline__ float safe_rcp(float x) {
+    return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x);
+}
+
+__device__ __forceinline__ float pick(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2(1), 2013),
+// restated identically by the CPU oracle so both agree on the hit bit for
+// bit: t in [0, tmax], barycentrics (b1, b2) on v1 and v2.
+struct WRay {
+    v3 o;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WRay make_wray(v3 o, v3 d) {
+    WRay w;
+    w.o = o;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dz = pick(d, kz);
+    if (dz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.Sx = pick(d, kx) / dz;
+    w.Sy = pick(d, ky) / dz;
+    w.Sz = 1.0f / dz;
+    return w;
+}
+
+__device__ __forceinline__ bool hit_triangle(const WRay& w, v3 v0, v3 v1, v3 v2, float tmax, float* t_out,
+                                             float* b1, float* b2) {
+    const v3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
+    float Az = pick(A, w.kz), Bz = pick(B, w.kz), Cz = pick(C, w.kz);
+    const float Ax = pick(A, w.kx) - w.Sx * Az, Ay = pick(A, w.ky) - w.Sy * Az;
+    const float Bx = pick(B, w.kx) - w.Sx * Bz, By = pick(B, w.ky) - w.Sy * Bz;
+    const float Cx = pick(C, w.kx) - w.Sx * Cz, Cy = pick(C, w.ky) - w.Sy * Cz;
+    float U = Cx * By - Cy * Bx;
+    float Vv = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || Vv == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        Vv = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
+    const float det = U + Vv + W;
+    if (det == 0.0f) return false;
+    Az = w.Sz * Az;
+    Bz = w.Sz * Bz;
+    Cz = w.Sz * Cz;
+    const float T = U * Az + Vv * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= 0.0f) || !(t <= tmax)) return false;
+    *t_out = t;
+    *b1 = Vv * rcp;
+    *b2 = W * rcp;
+    return true;
+}
+
+struct TraceHit {
+    bool hit;
+    float t;
+    uint32_t inst, prim;
+    float b1, b2;
+};
+
+enum class TraceKind { Closest, Shadow };
+
+struct ShadowQuery {
+    uint32_t target_inst, target_prim;
+    float t_early;      // an opaque non-target hit nearer than this occludes outright
+    bool occluded;      // set when such a hit, or a non-target closest hit, was found
+    float attenuation;  // product over fractional-opacity crossings in front of the end
+};
+
+struct OpacityQuery {
+    uint32_t pixel, sample, bounce, crossings;
+};
+
+// Closest hit over the BVH4 within [0, tmax] (intersect.slang:64-88,
+// trace_shadow.slang:50-72 — the driver traversal both call, rebuilt).
+template <TraceKind KIND>
+__device__ __forceinline__ TraceHit traverse(const SceneView& s, v3 o, v3 d, float tmax, uint32_t mask, Stack& st,
+                                             OpacityQuery* oq, ShadowQuery* sq, uint32_t* nodes_visited,
+                                             uint32_t* tris_tested) {
+    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    const WRay w = make_wray(o, d);
+    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
+    // Fractional-opacity crossings seen by a shadow ray (rare: only scenes
+    // with opacity < 1), kept to multiply in increasing t at the end.
+    float ct[16], ctr[16];
+    int ncross = 0;
+    float overflow = 1.0f;
+    st.sp = 0;
+    uint32_t cur = 0;
+    uint32_t nv = 0, nt = 0;
+    while (true) {
+        if (!(cur & CHILD_LEAF)) {
+            const BvhNode4 n = s.nodes[cur];
+            ++nv;
+            const float lx[4] = {n.lo_x.x, n.lo_x.y, n.lo_x.z, n.lo_x.w};
+            const float hx[4] = {n.hi_x.x, n.hi_x.y, n.hi_x.z, n.hi_x.w};
+            const float ly[4] = {n.lo_y.x, n.lo_y.y, n.lo_y.z, n.lo_y.w};
+            const float hy[4] = {n.hi_y.x, n.hi_y.y, n.hi_y.z, n.hi_y.w};
+            const float lz[4] = {n.lo_z.x, n.lo_z.y, n.lo_z.z, n.lo_z.w};
+            const float hz[4] = {n.hi_z.x, n.hi_z.y, n.hi_z.z, n.hi_z.w};
+            const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+            uint32_t hitc[4];
+            float hitd[4];
+            int nh = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float ax = fmaf(lx[c], ix, -ox), bx = fmaf(hx[c], ix, -ox);
+                const float ay = fmaf(ly[c], iy, -oy), by = fmaf(hy[c], iy, -oy);
+                const float az = fmaf(lz[c], iz, -oz), bz = fmaf(hz[c], iz, -oz);
+                const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+                const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), best.t)) * 1.0000004f;
+                if (t0 <= t1 && ch[c] != CHILD_EMPTY) {
+                    // insertion into the near-to-far list
+                    int k = nh++;
+                    while (k > 0 && hitd[k - 1] > t0) {
+                        hitd[k] = hitd[k - 1];
+                        hitc[k] = hitc[k - 1];
+                        --k;
+                    }
+                    hitd[k] = t0;
+                    hitc[k] = ch[c];
+                }
+            }
+            if (nh == 0) {
+                if (st.sp == 0) break;
+                cur = st.pop();
+                continue;
+            }
+            for (int k = nh - 1; k > 0; --k) st.push(hitc[k]);
+            cur = hitc[0];
+            continue;
+        }
+        // leaf
+        const uint32_t first = cur & LEAF_FIRST_MASK;
+        const uint32_t count = ((cur >> 27) & 15u) + 1u;
+        for (uint32_t k = 0; k < count; ++k) {
+            const BvhTriangle tri = s.tris[first + k];
+            ++nt;
+            const uint32_t flags = __float_as_uint(tri.v2.w);
+            if (((flags >> 8) & mask) == 0) continue;
+            float t, b1, b2;
+            if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), best.t, &t, &b1, &b2)) continue;
+            const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
+            if (best.hit && t == best.t && (inst > best.inst || (inst == best.inst && prim > best.prim))) continue;
+            if (KIND == TraceKind::Closest) {
+                if (flags & TRI_NON_OPAQUE) {
+                    const float opacity = s.materials[inst].opacity;
+                    const bool commit = oq->crossings >= TRANSPARENT_CAP ||
+                                        hash_uniform(oq->pixel, oq->sample, oq->bounce, oq->crossings) < opacity;
+                    ++oq->crossings;
+                    if (!commit) continue;
+                }
+            } else {
+                const bool target = inst == sq->target_inst && prim == sq->target_prim;
+                if (!target) {
+                    if (flags & TRI_NON_OPAQUE) {
+                        const float tr = 1.0f - s.materials[inst].opacity;
+                        if (ncross < 16) {
+                            ct[ncross] = t;
+                            ctr[ncross] = tr;
+                            ++ncross;
+                        } else {
+                            overflow *= tr;
+                        }
+                        continue;
+                    }
+                    if (t < sq->t_early) {
+                        sq->occluded = true;
+                        *nodes_visited += nv;
+                        *tris_tested += nt;
+                        return best;
+                    }
+                }
+            }
+            best.hit = true;
+            best.t = t;
+            best.inst = inst;
+            best.prim = prim;
+            best.b1 = b1;
+            best.b2 = b2;
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    *nodes_visited += nv;
+    *tris_tested += nt;
+    if (KIND == TraceKind::Shadow) {
+        const bool target_hit = best.hit && best.inst == sq->target_inst && best.prim == sq->target_prim;
+        if (best.hit && !target_hit) {
+            sq->occluded = true;
+            return best;
+        }
+        const float tEnd = best.hit ? best.t : tmax;
+        // insertion sort by (t, transparency), then multiply near to far
+        for (int i = 1; i < ncross; ++i) {
+            const float kt = ct[i], kr = ctr[i];
+            int j = i - 1;
+            while (j >= 0 && (ct[j] > kt || (ct[j] == kt && ctr[j] > kr))) {
+                ct[j + 1] = ct[j];
+                ctr[j + 1] = ctr[j];
+                --j;
+            }
+            ct[j + 1] = kt;
+            ctr[j + 1] = kr;
+        }
+        float att = 1.0f;
+        for (int i = 0; i < ncross; ++i) {
+            if (ct[i] < tEnd) att *= ctr[i];
+        }
+        sq->attenuation = att * overflow;
+    }
+    return best;
+}
+
+// --- raygen (raygen.slang:64-112) --------------------------------------------------
+
+__device__ __forceinline__ void camera_ray(const CameraView& cam, uint32_t width, uint32_t height, uint32_t pixel,
+                                           uint32_t sample, v3* origin, v3* direction, float* cos_forward) {
+    const v3 right = V3(cam.right.x, cam.right.y, cam.right.z);
+    const v3 up = V3(cam.up.x, cam.up.y, cam.up.z);
+    const v3 forward = V3(cam.forward.x, cam.forward.y, cam.forward.z);
+    const f2 jitter = sample_2d(pixel, sample, DIM_CAMERA_JITTER);
+    const float x = float(pixel % width) + jitter.x;
+    const float y = float(pixel / width) + jitter.y;
+    const float u = x / float(width), v = y / float(height);
+    const float ndcx = 2.0f * u - 1.0f, ndcy = 1.0f - 2.0f * v;
+    const v3 target = (forward + ndcx * right) + ndcy * up;
+    v3 o = V3(cam.position.x, cam.position.y, cam.position.z);
+    v3 d = normalize(target);
+    if (cam.aperture > 0.0f) {
+        // Shirley-Chiu concentric disk (raygen.slang:47-62)
+        const f2 l = sample_2d(pixel, sample, DIM_CAMERA_LENS);
+        const float ox = 2.0f * l.x - 1.0f, oy = 2.0f * l.y - 1.0f;
+        float dx = 0.0f, dy = 0.0f;
+        if (!(ox == 0.0f && oy == 0.0f)) {
+            float r, theta;
+            if (fabsf(ox) > fabsf(oy)) {
+                r = ox;
+                theta = (PI_K / 4.0f) * (oy / ox);
+            } else {
+                r = oy;
+                theta = (PI_K / 2.0f) - (PI_K / 4.0f) * (ox / oy);
+            }
+            dx = r * cosf(theta);
+            dy = r * sinf(theta);
+        }
+        dx = cam.aperture * dx;
+        dy = cam.aperture * dy;
+        const v3 offset = dx * normalize(right) + dy * normalize(up);
+        o = o + offset;
+        d = normalize(target - offset);
+    }
+    *origin = o;
+    *direction = d;
+    *cos_forward = dot(d, normalize(forward));
+}
+
+// One thread per pixel of the range. Also zero-fills the wave's per-pixel
+// accumulators (the reference's Pass::Fill, wavefront.rs:733-759) and
+// resets the range's queue counters: raygen owns every pixel exactly once.
+__global__ __launch_bounds__(BLOCK) void k_raygen(CameraView cam, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                  uint32_t counter_count, uint32_t width, uint32_t height,
+                                                  uint32_t base, uint32_t count, uint32_t sample) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < counter_count) counters[i] = 0;
+    if (i >= count) return;
+    const uint32_t pixel = base + i;
+    v3 o, d;
+    float cf;
+    camera_ray(cam, width, height, pixel, sample, &o, &d, &cf);
+    p.origin[i] = F4(o, 0.0f);
+    p.direction[i] = F4(d, cf);
+    p.throughput[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    tg.radiance[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tg.albedo) {
+        tg.albedo[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tg.normal[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tg.depth[pixel] = 0.0f;
+    }
+}
+
+// --- intersect (intersect.slang:45-88) ----------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_intersect(SceneView s, PathPool p, uint32_t* counters, uint32_t base,
+                                                     uint32_t range_count, uint32_t bounce, uint32_t sample,
+                                                     int counting) {
+    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
+    const uint32_t n = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
+    const uint32_t* rays = p.ray_queue[bounce & 1];
+    const uint32_t lanes = gridDim.x * BLOCK;
+    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
+    const uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
+    uint32_t nodes = 0, tris = 0;
+    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
+        const uint32_t i = first + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t path = 0;
+        bool hit = false;
+        if (valid) {
+            path = bounce == 0 ? i : rays[i];
+            const v3 o = xyz(p.origin[path]);
+            const v3 d = xyz(p.direction[path]);
+            OpacityQuery oq{base + path, sample, bounce, 0};
+            const TraceHit h = traverse<TraceKind::Closest>(s, o, d, 1e30f, mask, st, &oq, nullptr, &nodes, &tris);
+            hit = h.hit;
+            if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+        }
+        const uint32_t hs = wave_claim(&counters[bounce * Q_COUNT + Q_HIT], valid && hit);
+        const uint32_t ms = wave_claim(&counters[bounce * Q_COUNT + Q_MISS], valid && !hit);
+        if (valid) {
+            if (hit) p.hit_queue[hs] = path;
+            else p.miss_queue[ms] = path;
+        }
+    }
+    if (counting) {
+        wave_add_u64(&p.traversal_counts[0], nodes);
+        wave_add_u64(&p.traversal_counts[1], tris);
+    }
+}
+
+// --- shade_miss (shade_miss.slang:27-75) ----------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                      uint32_t base, uint32_t bounce, uint32_t light_sampling) {
+    const uint32_t n = counters[bounce * Q_COUNT + Q_MISS];
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const uint32_t path = p.miss_queue[i];
+        const uint32_t pixel = base + path;
+        const v3 direction = xyz(p.direction[path]);
+        const float4 thr = p.throughput[path];
+        float weight = 1.0f;
+        if (thr.w > 0.0f) {
+            if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
+                weight = 0.0f;
+            } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
+                const float neePdf = s.env_select_prob * env_pdf(s, direction);
+                weight = power_heuristic(thr.w, neePdf);
+            }
+        }
+        if (tg.albedo) {
+            const bool cameraRay = thr.w == 0.0f;
+            if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
+            const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
+            add_pixel(tg.albedo, pixel, guide, 0.0f);
+        }
+        const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
+        add_pixel(tg.radiance, pixel, contribution, 1.0f);
+    }
+}
+
+// --- shade_surface (shade_surface.slang:233-521) --------------------------------------
+
+struct Spawn {
+    v3 position, below, normal, shading;
+    bool frontFace;
+};
+
+__device__ __forceinline__ v3 load3(const float* a, uint32_t v) { return V3(a[3 * v], a[3 * v + 1], a[3 * v + 2]); }
+
+// shade_surface.slang:96-181: the hit rebuilt from the object-space mesh,
+// offset along the geometric normal by a rigorous bound on its own
+// reconstruction error (van Antwerpen 2023).
+__device__ inline Spawn spawn_point(const SceneView& s, const InstanceRecord& inst, uint32_t prim, float bx, float by,
+                                    v3 rayDirection) {
+    const uint32_t* idx = s.indices + 3u * (inst.index_base + prim);
+    const uint32_t i0 = inst.vertex_base + idx[0], i1 = inst.vertex_base + idx[1], i2 = inst.vertex_base + idx[2];
+    const v3 v0 = load3(s.positions, i0), v1 = load3(s.positions, i1), v2 = load3(s.positions, i2);
+    const float4 a0 = inst.o2w[0], a1 = inst.o2w[1], a2 = inst.o2w[2];
+    const float4 w0 = inst.w2o[0], w1 = inst.w2o[1], w2 = inst.w2o[2];
+
+    const v3 edge1 = v1 - v0, edge2 = v2 - v0;
+    const v3 objPosition = v0 + vfma(S3(bx), edge1, by * edge2);
+    const v3 objNormal = cross(edge1, edge2);
+    const v3 wldPosition = V3(a0.w + fmaf(a0.x, objPosition.x, fmaf(a0.y, objPosition.y, a0.z * objPosition.z)),
+                              a1.w + fmaf(a1.x, objPosition.x, fmaf(a1.y, objPosition.y, a1.z * objPosition.z)),
+                              a2.w + fmaf(a2.x, objPosition.x, fmaf(a2.y, objPosition.y, a2.z * objPosition.z)));
+    const v3 r0 = xyz(w0), r1 = xyz(w1), r2 = xyz(w2);
+    v3 wldNormal = (r0 * objNormal.x + r1 * objNormal.y) + r2 * objNormal.z;
+    const float wldScale = rsqrt_exact(dot(wldNormal, wldNormal));
+    wldNormal = wldNormal * wldScale;
+
+    const float c0 = 5.9604644775390625e-8f;
+    const float c1 = 1.788139769587360206060111522674560546875e-7f;
+    const float c2 = 1.19209317972490680404007434844970703125e-7f;
+    const v3 extent3 = (vabs(edge1) + vabs(edge2)) + vabs(edge1 - edge2);
+    const float extent = fmaxf(fmaxf(extent3.x, extent3.y), extent3.z);
+    v3 objErr = vfma(S3(c0), vabs(v0), S3(c1 * extent));
+    const v3 aobj = vabs(objPosition);
+    const v3 wldErr = vfma(S3(c1), V3(dot(vabs(xyz(a0)), aobj), dot(vabs(xyz(a1)), aobj), dot(vabs(xyz(a2)), aobj)),
+                           c2 * vabs(V3(a0.w, a1.w, a2.w)));
+    const v3 awld = vabs(wldPosition);
+    objErr = vfma(S3(c2),
+                  V3(dot(vabs(r0), awld) + fabsf(w0.w), dot(vabs(r1), awld) + fabsf(w1.w),
+                     dot(vabs(r2), awld) + fabsf(w2.w)),
+                  objErr);
+    float wldOffset = dot(wldErr, vabs(wldNormal));
+    const float objOffset = dot(objErr, vabs(objNormal));
+    wldOffset = fmaf(wldScale, objOffset, wldOffset);
+
+    Spawn sp;
+    sp.frontFace = dot(wldNormal, rayDirection) < 0.0f;
+    if (dot(wldNormal, rayDirection) > 0.0f) wldNormal = -wldNormal;
+    const v3 n0 = load3(s.normals, i0), n1 = load3(s.normals, i1), n2 = load3(s.normals, i2);
+    const v3 objShading = (n0 + bx * (n1 - n0)) + by * (n2 - n0);
+    v3 shading = normalize((r0 * objShading.x + r1 * objShading.y) + r2 * objShading.z);
+    if (dot(shading, wldNormal) < 0.0f) shading = -shading;
+    sp.position = vfma(S3(wldOffset), wldNormal, wldPosition);
+    sp.below = vfma(S3(-wldOffset), wldNormal, wldPosition);
+    sp.normal = wldNormal;
+    sp.shading = shading;
+    return sp;
+}
+
+__device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* recs, uint32_t i) {
+    return recs[i];
+}
+
+// The BSDF tables are staged into LDS once per workgroup — 67,712 bytes
+// that every closure evaluation of the block reads ~20 times — so the
+// workgroup is wide (1024 lanes, 16 waves) to amortise the staging and keep
+// a CU's LDS for two such groups.
+constexpr uint32_t SHADE_BLOCK = 1024;
+
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                         uint32_t base, uint32_t bounce, uint32_t max_bounces,
+                                                         uint32_t light_sampling, uint32_t sample) {
+    __shared__ float lds_tables[T_FLOATS];
+    const uint32_t n = counters[bounce * Q_COUNT + Q_HIT];
+    if (blockIdx.x * SHADE_BLOCK >= n) return;  // uniform per block: nothing to shade here
+    for (uint32_t k = threadIdx.x; k < T_FLOATS / 4; k += SHADE_BLOCK) {
+        reinterpret_cast<float4*>(lds_tables)[k] = reinterpret_cast<const float4*>(s.tables)[k];
+    }
+    __syncthreads();
+    const float* tables = lds_tables;
+    const bool aovOn = tg.albedo != nullptr;
+    uint32_t* next_rays = p.ray_queue[(bounce + 1) & 1];
+
+    for (uint32_t first = blockIdx.x * SHADE_BLOCK; first < n; first += gridDim.x * SHADE_BLOCK) {
+        const uint32_t i = first + threadIdx.x;
+        const bool valid = i < n;
+        bool push_ray = false, push_shadow = false;
+        uint32_t path = 0, pixel = 0;
+        v3 sh_origin = S3(0.0f), sh_dir = S3(0.0f), sh_contrib = S3(0.0f);
+        float sh_tmax = 0.0f;
+        uint32_t sh_light = LIGHT_NONE, sh_prim = 0;
+        if (valid) {
+            path = p.hit_queue[i];
+            pixel = base + path;
+            const uint4 hit = p.hit[path];
+            const float4 dir4 = p.direction[path];
+            const v3 direction = xyz(dir4);
+            const v3 origin = xyz(p.origin[path]);
+            const InstanceRecord inst = load_instance(s.instances, hit.x);
+            const Mat material = load_material(s.materials, hit.x);
+            const Spawn spawn = spawn_point(s, inst, hit.y, __uint_as_float(hit.z), __uint_as_float(hit.w), direction);
+            float4 thr = p.throughput[path];
+
+            uint32_t medium = bounce == 0 ? MEDIUM_NONE : (p.state[path] & 0xffffffu);
+            if (medium != MEDIUM_NONE) {
+                const Mat interior = load_material(s.materials, medium);
+                if (interior.transmissionDepth > 0.0f) {
+                    const v3 tc = vclamp(interior.transmissionColor, 1e-4f, 1.0f);
+                    const v3 sigma = (-V3(logf(tc.x), logf(tc.y), logf(tc.z))) / interior.transmissionDepth;
+                    const v3 e = -sigma * length(origin - spawn.position);
+                    thr.x *= expf(e.x);
+                    thr.y *= expf(e.y);
+                    thr.z *= expf(e.z);
+                }
+            }
+            const v3 T = xyz(thr);
+
+            v3 contribution = S3(0.0f);
+            if (inst.light != LIGHT_NONE && spawn.frontFace) {
+                float weight = 1.0f;
+                if (thr.w > 0.0f) {
+                    if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
+                        weight = 0.0f;
+                    } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
+                        const v3 delta = spawn.position - origin;
+                        const float cosLight = fabsf(dot(spawn.normal, direction));
+                        const float neePdf = (1.0f - s.env_select_prob) * s.lights[inst.light + hit.y].pdf *
+                                             dot(delta, delta) / cosLight;
+                        weight = power_heuristic(thr.w, neePdf);
+                    }
+                }
+                contribution = (T * material.emission) * weight;
+            }
+
+            const v3 shading = spawn.shading;
+            const Frame frame = make_frame(shading);
+            const v3 wo = to_local(frame, -direction);
+            const Closure closure = resolve_closure(tables, material, wo, medium == hit.x);
+
+            v3 guide = S3(0.0f);
+            if (aovOn) {
+                guide = bounce == 0 ? S3(1.0f) : xyz(tg.guide[pixel]);
+                if (bounce == 0) tg.depth[pixel] += length(origin - spawn.position) * dir4.w;
+            }
+            const v3 aovAlbedo = vclamp(closure.aovAlbedo, 0.0f, 1.0f);
+            bool terminal = false;
+
+            if (bounce + 1 >= max_bounces) {
+                terminal = true;
+            } else {
+                // Next-event estimation (shade_surface.slang:363-446).
+                const float envProb = s.env_select_prob;
+                if (light_sampling != CN_LIGHT_SAMPLING_BSDF && (s.light_count > 0 || envProb > 0.0f)) {
+                    const float uLight = sample_1d(pixel, sample, bounce_dim(bounce, SLOT_NEE_LIGHT));
+                    const f2 uPoint = sample_2d(pixel, sample, bounce_dim(bounce, SLOT_NEE_POINT));
+                    v3 unshadowed = S3(0.0f);
+                    if (uLight < envProb) {
+                        const EnvSample cand = env_sample(s, uPoint);
+                        const v3 wiLight = to_local(frame, cand.direction);
+                        const bool through = dot(cand.direction, spawn.normal) < 0.0f;
+                        if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
+                            const float neePdf = envProb * cand.pdf;
+                            const float scatterPdf = bsdf_pdf(closure, wiLight, wo);
+                            const float weight =
+                                light_sampling == CN_LIGHT_SAMPLING_MIS ? power_heuristic(neePdf, scatterPdf) : 1.0f;
+                            unshadowed = (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
+                                         (weight / neePdf);
+                            sh_origin = through ? spawn.below : spawn.position;
+                            sh_light = LIGHT_NONE;
+                            sh_prim = 0;
+                            sh_dir = cand.direction;
+                            sh_tmax = 1e30f;
+                        }
+                    } else {
+                        const float uList = (uLight - envProb) / (1.0f - envProb);
+                        const LightConnection cand = sample_lights(s, uList, uPoint, spawn.position);
+                        const v3 wiLight = to_local(frame, cand.direction);
+                        const bool through = dot(cand.direction, spawn.normal) < 0.0f;
+                        if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
+                            const float neePdf = (1.0f - envProb) * cand.pdf;
+                            const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
+                                                     ? power_heuristic(neePdf, bsdf_pdf(closure, wiLight, wo))
+                                                     : 1.0f;
+                            unshadowed = (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
+                                         (weight / neePdf);
+                            sh_origin = through ? spawn.below : spawn.position;
+                            sh_light = cand.instance;
+                            sh_prim = cand.primitive;
+                            sh_dir = cand.direction;
+                            sh_tmax = cand.tmax;
+                        }
+                    }
+                    if (any_positive(unshadowed)) {
+                        push_shadow = true;
+                        sh_contrib = unshadowed;
+                    }
+                }
+
+                // Scatter (shade_surface.slang:453-489).
+                const float uLobe = sample_1d(pixel, sample, bounce_dim(bounce, SLOT_BSDF_LOBE));
+                const f2 u = sample_2d(pixel, sample, bounce_dim(bounce, SLOT_BSDF_DIR));
+                const BsdfSample scatter = sample_bsdf(closure, wo, uLobe, u);
+                const v3 wiWorld = to_world(frame, scatter.wi);
+                v3 scattered = S3(0.0f);
+                if (scatter.pdf > 0.0f && scatter.wi.z != 0.0f &&
+                    (scatter.wi.z > 0.0f) == (dot(wiWorld, spawn.normal) > 0.0f)) {
+                    scattered = (T * eval_bsdf(closure, scatter.wi, wo)) * (fabsf(scatter.wi.z) / scatter.pdf);
+                }
+                float survival = fmaxf(scattered.x, fmaxf(scattered.y, scattered.z));
+                if (survival <= 0.0f) {
+                    terminal = true;
+                } else {
+                    if (bounce >= ROULETTE_START) {
+                        survival = fminf(1.0f, survival);
+                        if (sample_1d(pixel, sample, bounce_dim(bounce, SLOT_RR)) >= survival) terminal = true;
+                        else scattered = scattered / survival;
+                    }
+                }
+                if (!terminal) {
+                    if (aovOn) {
+                        const float record = aov_record_fraction(closure, scatter.lobe);
+                        if (record > 0.0f && any_positive(guide)) {
+                            const v3 g = guide * record;
+                            add_pixel(tg.albedo, pixel, g * aovAlbedo, 0.0f);
+                            add_pixel(tg.normal, pixel, g * shading, 0.0f);
+                        }
+                        tg.guide[pixel] = F4((guide * (1.0f - record)) * aov_pass_tint(closure, scatter.lobe), 0.0f);
+                    }
+                    if (scatter.lobe == LOBE_TRANSMISSION && material.thinWalled == 0) {
+                        medium = medium == hit.x ? MEDIUM_NONE : hit.x;
+                    }
+                    const bool transmitted = scatter.wi.z < 0.0f;
+                    p.origin[path] = F4(transmitted ? spawn.below : spawn.position, 0.0f);
+                    p.direction[path] = F4(wiWorld, 0.0f);
+                    p.throughput[path] = F4(scattered, scatter.pdf);
+                    p.state[path] = (scatter.lobe << 24) | (medium & 0xffffffu);
+                    push_ray = true;
+                    if (any_positive(contribution)) add_pixel(tg.radiance, pixel, contribution, 0.0f);
+                }
+            }
+            if (terminal) {
+                if (aovOn && any_positive(guide)) {
+                    add_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f);
+                    add_pixel(tg.normal, pixel, guide * shading, 0.0f);
+                }
+                add_pixel(tg.radiance, pixel, contribution, 1.0f);
+            }
+        }
+        const uint32_t rs = wave_claim(&counters[(bounce + 1) * Q_COUNT + Q_RAY], push_ray);
+        const uint32_t ss = wave_claim(&counters[bounce * Q_COUNT + Q_SHADOW], push_shadow);
+        if (push_ray) next_rays[rs] = path;
+        if (push_shadow) {
+            p.shadow_origin[ss] = F4(sh_origin, sh_tmax);
+            p.shadow_dir[ss] = F4(sh_dir, __uint_as_float(pixel));
+            p.shadow_contrib[ss] = F4(sh_contrib, __uint_as_float(sh_light));
+            p.shadow_prim[ss] = sh_prim;
+        }
+    }
+}
+
+// --- trace_shadow (trace_shadow.slang:27-72) ------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_trace_shadow(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                        uint32_t bounce, int counting) {
+    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
+    const uint32_t n = counters[bounce * Q_COUNT + Q_SHADOW];
+    const uint32_t lanes = gridDim.x * BLOCK;
+    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
+    uint32_t nodes = 0, tris = 0;
+    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
+        const uint32_t i = first + threadIdx.x;
+        if (i >= n) continue;
+        const float4 o4 = p.shadow_origin[i];
+        const float4 d4 = p.shadow_dir[i];
+        const float4 c4 = p.shadow_contrib[i];
+        ShadowQuery sq;
+        sq.target_inst = __float_as_uint(c4.w);
+        sq.target_prim = p.shadow_prim[i];
+        // Any solid non-target hit short of the light's own distance
+        // (tmax / 1.001 for a triangle target) must lie in front of the
+        // target, so it occludes without finding the closest hit.
+        sq.t_early = sq.target_inst == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
+        sq.occluded = false;
+        sq.attenuation = 1.0f;
+        traverse<TraceKind::Shadow>(s, xyz(o4), xyz(d4), o4.w, MASK_ALL, st, nullptr, &sq, &nodes, &tris);
+        if (sq.occluded || sq.attenuation < 1e-4f) continue;
+        add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * sq.attenuation, 0.0f);
+    }
+    if (counting) {
+        wave_add_u64(&p.traversal_counts[2], nodes);
+        wave_add_u64(&p.traversal_counts[3], tris);
+    }
+}
+
+// --- film (accumulate.slang:46-67, resolve.slang:31-42) ----------------------------
+
+__device__ __forceinline__ float4 guarded(float4 c) {
+    const bool bad = isnan(c.x) || isnan(c.y) || isnan(c.z) || isnan(c.w) || isinf(c.x) || isinf(c.y) ||
+                     isinf(c.z) || isinf(c.w);
+    return bad ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : c;
+}
+
+__device__ __forceinline__ float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__global__ __launch_bounds__(BLOCK) void k_accumulate(FilmView f, uint32_t pixels, uint32_t reset) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pixels; i += gridDim.x * BLOCK) {
+        const float4 b = guarded(f.beauty[i]);
+        f.beauty_sum[i] = reset ? b : add4(f.beauty_sum[i], b);
+        const float4 a = guarded(f.albedo[i]);
+        f.albedo_sum[i] = reset ? a : add4(f.albedo_sum[i], a);
+        const float4 nn = guarded(f.normal[i]);
+        f.normal_sum[i] = reset ? nn : add4(f.normal_sum[i], nn);
+        float d = f.depth[i];
+        if (isnan(d)) d = 0.0f;
+        f.depth_sum[i] = reset ? d : f.depth_sum[i] + d;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_resolve(FilmView f, uint32_t pixels, float samples) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pixels; i += gridDim.x * BLOCK) {
+        const float4 b = f.beauty_sum[i], a = f.albedo_sum[i], nn = f.normal_sum[i];
+        f.beauty[i] = make_float4(b.x / samples, b.y / samples, b.z / samples, b.w / samples);
+        f.albedo[i] = make_float4(a.x / samples, a.y / samples, a.z / samples, a.w / samples);
+        f.normal[i] = make_float4(nn.x / samples, nn.y / samples, nn.z / samples, nn.w / samples);
+        f.depth[i] = f.depth_sum[i] / samples;
+    }
+}
+
+// --- diagnostics ----------------------------------------------------------------------
+
+// rng_test.slang:21-29: the first `count` points of one (pixel, dimension) key.
+__global__ void k_sample_sequence(uint32_t pixel, uint32_t dimension, uint32_t count, float* points, float* values) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const f2 pt = sample_2d(pixel, i, dimension);
+    points[2 * i] = pt.x;
+    points[2 * i + 1] = pt.y;
+    values[i] = sample_1d(pixel, i, dimension);
+}
+
+// The bounce-0 closest hit per pixel, for the first-hit identity check.
+__global__ __launch_bounds__(BLOCK) void k_first_hits(SceneView s, CameraView cam, PathPool p, uint32_t width,
+                                                      uint32_t height, uint32_t sample, uint32_t* inst_out,
+                                                      uint32_t* prim_out, float* bary_out) {
+    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
+    const uint32_t n = width * height;
+    const uint32_t lanes = gridDim.x * BLOCK;
+    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
+    uint32_t nodes = 0, tris = 0;
+    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
+        const uint32_t pixel = first + threadIdx.x;
+        if (pixel >= n) continue;
+        v3 o, d;
+        float cf;
+        camera_ray(cam, width, height, pixel, sample, &o, &d, &cf);
+        OpacityQuery oq{pixel, sample, 0, 0};
+        const TraceHit h = traverse<TraceKind::Closest>(s, o, d, 1e30f, MASK_CAMERA, st, &oq, nullptr, &nodes, &tris);
+        inst_out[pixel] = h.hit ? h.inst : 0xffffffffu;
+        prim_out[pixel] = h.hit ? h.prim : 0xffffffffu;
+        bary_out[2 * pixel] = h.hit ? h.b1 : 0.0f;
+        bary_out[2 * pixel + 1] = h.hit ? h.b2 : 0.0f;
+    }
+}
+
+// --- launchers --------------------------------------------------------------------------
+
+static uint32_t grid_for(uint32_t items, uint32_t cap) {
+    uint32_t blocks = (items + BLOCK - 1) / BLOCK;
+    if (blocks < 1) blocks = 1;
+    return blocks < cap ? blocks : cap;
+}
+
+hipError_t launch_raygen(hipStream_t stream, const CameraView& cam, const PathPool& p, const WaveTargets& tg,
+                         uint32_t* counters, uint32_t counter_count, uint32_t width, uint32_t height, uint32_t base,
+                         uint32_t count, uint32_t sample, Profiler* prof) {
+    const uint32_t items = count > counter_count ? count : counter_count;
+    if (prof) prof->begin(K_RAYGEN);
+    hipLaunchKernelGGL(k_raygen, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, cam, p, tg, counters,
+                       counter_count, width, height, base, count, sample);
+    if (prof) prof->end(K_RAYGEN);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
+                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, uint32_t sample,
+                         int counting, Profiler* prof) {
+    const uint32_t trace_grid = shape.trace_blocks;
+    uint32_t shade_grid = (range_count + SHADE_BLOCK - 1) / SHADE_BLOCK;
+    if (shade_grid > shape.shade_blocks) shade_grid = shape.shade_blocks;
+    if (shade_grid < 1) shade_grid = 1;
+    const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
+    if (prof) prof->begin(K_INTERSECT);
+    hipLaunchKernelGGL(k_intersect, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, counters, base, range_count,
+                       bounce, sample, counting);
+    if (prof) prof->end(K_INTERSECT);
+    if (prof) prof->begin(K_SHADE_MISS);
+    hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
+                       light_sampling);
+    if (prof) prof->end(K_SHADE_MISS);
+    if (prof) prof->begin(K_SHADE_SURFACE);
+    hipLaunchKernelGGL(k_shade_surface, dim3(shade_grid), dim3(SHADE_BLOCK), 0, stream, s, p, tg, counters, base,
+                       bounce, max_bounces, light_sampling, sample);
+    if (prof) prof->end(K_SHADE_SURFACE);
+    if (bounce + 1 < max_bounces) {
+        if (prof) prof->begin(K_TRACE_SHADOW);
+        hipLaunchKernelGGL(k_trace_shadow, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, bounce,
+                           counting);
+        if (prof) prof->end(K_TRACE_SHADOW);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t stream, const FilmView& f, uint32_t pixels, uint32_t reset, Profiler* prof) {
+    if (prof) prof->begin(K_ACCUMULATE);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(pixels, 4096)), dim3(BLOCK), 0, stream, f, pixels, reset);
+    if (prof) prof->end(K_ACCUMULATE);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(hipStream_t stream, const FilmView& f, uint32_t pixels, float samples) {
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 4096)), dim3(BLOCK), 0, stream, f, pixels, samples);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t dimension, uint32_t count,
+                                  float* points, float* values) {
+    hipLaunchKernelGGL(k_sample_sequence, dim3((count + 255) / 256), dim3(256), 0, stream, pixel, dimension, count,
+                       points, values);
+    return hipGetLastError();
+}
+
+hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const CameraView& cam,
+                             const PathPool& p, uint32_t width, uint32_t height, uint32_t sample, uint32_t* inst,
+                             uint32_t* prim, float* bary) {
+    hipLaunchKernelGGL(k_first_hits, dim3(shape.trace_blocks), dim3(BLOCK), 0, stream, s, cam, p, width, height, sample,
+                       inst, prim, bary);
+    return hipGetLastError();
+}
+
+}  // namespace cn

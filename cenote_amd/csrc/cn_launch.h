@@ -1,0 +1,72 @@
+// cn_launch.h — the host-callable launchers of the stage kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cn_records.h"
+
+namespace cn {
+
+// Traversal stack: STACK_LDS entries per lane in LDS, the rest (up to
+// STACK_MAX) in the per-lane HBM spill area. The BVH builder rejects trees
+// whose closest-first traversal could need more than STACK_MAX entries
+// (3 per level of a 4-wide tree).
+constexpr uint32_t STACK_LDS = 16;
+constexpr uint32_t STACK_MAX = 128;
+constexpr uint32_t STACK_SPILL = STACK_MAX - STACK_LDS;
+
+// Grid sizes: the traversal kernels run a fixed, chip-filling grid and
+// walk their queue; the shading kernels are capped likewise.
+struct LaunchShape {
+    uint32_t trace_blocks;  // blocks of 256 lanes for intersect / trace_shadow
+    uint32_t shade_blocks;  // blocks of 1024 lanes for shade_surface
+    uint32_t light_blocks;  // blocks of 256 lanes for shade_miss
+};
+
+// The film's buffers (render/film.rs:66-127): one wave's sample
+// accumulators and the running sums.
+struct FilmView {
+    float4* beauty;
+    float4* beauty_sum;
+    float4* albedo;
+    float4* albedo_sum;
+    float4* normal;
+    float4* normal_sum;
+    float* depth;
+    float* depth_sum;
+};
+
+// The stage kernels, for per-kernel timing.
+enum KernelKind { K_RAYGEN = 0, K_INTERSECT, K_SHADE_MISS, K_SHADE_SURFACE, K_TRACE_SHADOW, K_ACCUMULATE, K_KINDS };
+
+// Brackets each launch (the renderer records HIP events on the launch
+// stream when profiling is on).
+struct Profiler {
+    virtual void begin(int kind) = 0;
+    virtual void end(int kind) = 0;
+    virtual ~Profiler() = default;
+};
+
+hipError_t launch_raygen(hipStream_t stream, const CameraView& cam, const PathPool& p, const WaveTargets& tg,
+                         uint32_t* counters, uint32_t counter_count, uint32_t width, uint32_t height, uint32_t base,
+                         uint32_t count, uint32_t sample, Profiler* prof);
+
+hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
+                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, uint32_t sample,
+                         int counting, Profiler* prof);
+
+hipError_t launch_accumulate(hipStream_t stream, const FilmView& f, uint32_t pixels, uint32_t reset, Profiler* prof);
+
+// Divides the sums into the FilmView's sample-buffer slots (the caller
+// passes the resolve targets there).
+hipError_t launch_resolve(hipStream_t stream, const FilmView& f, uint32_t pixels, float samples);
+
+hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t dimension, uint32_t count,
+                                  float* points, float* values);
+
+hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const CameraView& cam,
+                             const PathPool& p, uint32_t width, uint32_t height, uint32_t sample, uint32_t* inst,
+                             uint32_t* prim, float* bary);
+
+}  // namespace cn

@@ -1,0 +1,58 @@
+// cn_prep.h — host-side scene preparation: the light table, the environment
+// sampling tables, transforms and the camera basis. Everything here runs
+// once per scene on the host, in the operation order the reference's host
+// code uses, because its outputs (alias tables, CDFs) must match bit for bit.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/cenote_hip.h"
+#include "cn_math.h"
+#include "cn_records.h"
+
+namespace cn {
+
+// An emissive triangle in world space (lights.rs:39-52).
+struct EmissiveTriangle {
+    v3 corners[3];
+    v3 emission;
+    uint32_t instance, primitive;
+};
+
+// color.rs:29-31
+float luminance(v3 c);
+
+// lights.rs:175-224, :229-231. Returns the records; *total_power is the
+// list's total selection weight.
+std::vector<cn_light_record> build_light_table(const std::vector<EmissiveTriangle>& tris,
+                                               const cn_delta_light* deltas, uint32_t delta_count,
+                                               double* total_power);
+
+struct EnvironmentTables {
+    std::vector<float> marginal;     // height + 1
+    std::vector<float> conditional;  // height * (width + 1)
+    std::vector<float> pdfs;         // height * width
+    double power = 0.0;
+};
+
+// environment.rs:130-218
+EnvironmentTables build_environment_tables(uint32_t width, uint32_t height, const float* texels);
+
+// scene/mod.rs:665-671
+float select_probability(double env_power, double light_power);
+
+// glam's Mat4::transform_point3 over the rows of a 3x4 affine transform.
+v3 transform_point(const float rows[12], v3 p);
+
+// The inverse of a 3x4 affine transform, computed in f64 and rounded.
+// Returns false when the transform is singular or not finite.
+bool invert_affine(const float rows[12], float out[12]);
+
+// Camera::basis (scene/mod.rs:448-461) for `aspect`, with the thin-lens
+// pre-scaling of wavefront.rs:643-653 and the focus default of
+// scene/lower.rs:227-232. Returns false on a degenerate camera.
+bool camera_view(const cn_camera& camera, float aspect, CameraView* out);
+
+}  // namespace cn

@@ -1,0 +1,152 @@
+// cn_records.h — the records the stage kernels read from HBM, shared by the
+// host (which builds them) and the device (which reads them).
+//
+// Layouts are chosen for gfx950: every record is a whole number of 16-byte
+// lanes so a wave64 fetches it with dwordx4 loads, and the per-path state is
+// structure-of-arrays so consecutive lanes touch consecutive addresses.
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cenote_hip.h"
+
+namespace cn {
+
+// A stage queue is one u32 counter per (range, bounce, queue); the queues
+// themselves are entry arrays sized to the path pool.
+enum Queue : uint32_t { Q_RAY = 0, Q_HIT = 1, Q_MISS = 2, Q_SHADOW = 3, Q_COUNT = 4 };
+
+// Most pixel ranges a single wave may walk (pixels / capacity, rounded up).
+constexpr uint32_t MAX_RANGES = 64;
+// Counters one range holds: 4 queues per bounce, plus one spare bounce.
+constexpr uint32_t counters_per_range(uint32_t max_bounces) { return (max_bounces + 1) * Q_COUNT; }
+
+// Floats in the BSDF table blob (crates/cenote/src/tables.rs:67-80).
+constexpr uint32_t BSDF_TABLE_FLOATS = 16928;
+
+constexpr uint32_t LIGHT_NONE = 0xffffffffu;    // shaders/lights.slang:20
+constexpr uint32_t MEDIUM_NONE = 0xffffffu;     // shaders/pathstate.slang:188
+constexpr uint32_t MASK_CAMERA = 0x01u;         // scene/mod.rs:54-57
+constexpr uint32_t MASK_ALL = 0xffu;
+constexpr uint32_t LIGHT_KIND_TRIANGLE = 0;     // shaders/lights.slang:23-25
+constexpr uint32_t LIGHT_KIND_DISTANT = 1;
+constexpr uint32_t LIGHT_KIND_POINT = 2;
+
+// --- BVH -------------------------------------------------------------------
+//
+// A 4-wide BVH over world-space triangles (the flattened TLAS x BLAS of
+// gpu/accel.rs:82-163). 128 bytes per node: one L2 line on gfx950. The six
+// float4s hold the four children's slab bounds as structure-of-arrays, so
+// the four box tests are plain lane-local vector math. `child` encodes:
+//   bit 31 clear        -> internal node index
+//   bit 31 set          -> leaf: bits 0..26 first triangle, bits 27..30
+//                          count-1 (1..16 triangles)
+//   0xffffffff          -> empty slot (its box is inverted, never hit)
+struct alignas(16) BvhNode4 {
+    float4 lo_x, hi_x, lo_y, hi_y, lo_z, hi_z;
+    uint4 child;
+    uint4 meta;  // x = node depth; y..w unused
+};
+static_assert(sizeof(BvhNode4) == 128, "one 128-byte line per node");
+
+constexpr uint32_t CHILD_EMPTY = 0xffffffffu;
+constexpr uint32_t CHILD_LEAF = 0x80000000u;
+constexpr uint32_t LEAF_MAX = 16;
+constexpr uint32_t LEAF_FIRST_MASK = (1u << 27) - 1;
+
+__host__ __device__ inline uint32_t leaf_encode(uint32_t first, uint32_t count) {
+    return CHILD_LEAF | ((count - 1) << 27) | first;
+}
+
+// One triangle in leaf order, world space, 48 bytes:
+//   v0.w = instance (bits), v1.w = primitive (bits),
+//   v2.w = flags (bits): bit 0 set = non-opaque instance; bits 8..15 = mask.
+struct alignas(16) BvhTriangle {
+    float4 v0, v1, v2;
+};
+static_assert(sizeof(BvhTriangle) == 48, "three dwordx4 fetches per triangle");
+
+constexpr uint32_t TRI_NON_OPAQUE = 1u;
+
+// --- Scene records -----------------------------------------------------------
+
+// Per instance, 128 bytes: the GeometryRecord of scene/mod.rs:104-123 with
+// the mesh pointers replaced by offsets into the scene's pooled vertex and
+// index arrays.
+struct alignas(16) InstanceRecord {
+    float4 o2w[3];   // rows of the object-to-world transform
+    float4 w2o[3];   // rows of its inverse
+    uint32_t light;  // first light record, or LIGHT_NONE
+    uint32_t vertex_base;
+    uint32_t index_base;  // in triangles
+    uint32_t flags;       // bits 0..7 = visibility mask, bit 8 = opaque
+    uint32_t pad[4];
+};
+static_assert(sizeof(InstanceRecord) == 128, "two dwordx4 lines per instance");
+
+static_assert(sizeof(cn_material) == 144, "material.rs:116");
+static_assert(sizeof(cn_light_record) == 80, "lights.rs:115");
+
+// Everything the stage kernels read of a scene, passed by value.
+struct SceneView {
+    const BvhNode4* nodes;
+    const BvhTriangle* tris;
+    const InstanceRecord* instances;
+    const cn_material* materials;
+    const cn_light_record* lights;
+    const float* positions;  // pooled object-space xyz
+    const float* normals;    // pooled object-space xyz
+    const uint32_t* indices; // pooled, mesh-local vertex indices
+    const float* tables;     // BSDF tables (16,928 floats)
+    const float4* env_texels;
+    const float* env_marginal;
+    const float* env_conditional;
+    const float* env_pdfs;
+    uint32_t light_count;
+    uint32_t env_width;
+    uint32_t env_height;
+    float env_select_prob;
+};
+
+// The SoA path pool (shaders/pathstate.slang:207-232). A path's pixel is
+// its range base plus its slot, so no pixel field is stored.
+struct PathPool {
+    float4* origin;       // xyz origin
+    float4* direction;    // xyz direction, w = cosine to the camera forward
+    uint4* hit;           // instance, primitive, b1 bits, b2 bits
+    float4* throughput;   // xyz weight, w = previous scatter pdf
+    uint32_t* state;      // lobe << 24 | medium
+    uint32_t* ray_queue[2];  // ping-pong ray queues for bounces >= 1
+    uint32_t* hit_queue;
+    uint32_t* miss_queue;
+    // Shadow records (shaders/pathstate.slang:306-317) as SoA:
+    float4* shadow_origin;   // xyz, w = tmax
+    float4* shadow_dir;      // xyz, w = pixel (bits)
+    float4* shadow_contrib;  // xyz, w = target instance (bits)
+    uint32_t* shadow_prim;
+    uint32_t* stack_overflow;  // per-lane spill area of the traversal stack
+    unsigned long long* traversal_counts;  // 4 counters when enabled
+};
+
+// The wave's per-pixel targets (shaders/pathstate.slang:241-258 plus the
+// radiance accumulator).
+struct WaveTargets {
+    float4* radiance;
+    float4* albedo;  // NULL when AOVs are off
+    float4* normal;
+    float* depth;
+    float4* guide;
+};
+
+// The camera as raygen reads it (wavefront.rs:84-119 RaygenParams).
+struct CameraView {
+    float3 position;
+    float3 right;
+    float3 up;
+    float3 forward;
+    float aperture;
+};
+
+}  // namespace cn

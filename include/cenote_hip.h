@@ -1,0 +1,335 @@
+/*
+ * cenote_hip.h — the C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * A drop-in for the hot path of the `cenote` crate: scene residency
+ * (`Scene::new` / `Scene::prep`), the wavefront engine (`Wavefront::new`,
+ * `Wavefront::trace`), and the film (`Film`, `Renderer::accumulate`,
+ * `Renderer::resolve`, `Film::averages`). Every entry point takes plain
+ * pointers and sizes; no entry point exposes a device runtime type.
+ *
+ * Conventions kept from the reference (crates/cenote/src/lib.rs:37-38,
+ * shaders/raygen.slang:7): right-handed, Y-up, the camera looks -Z; images
+ * are row-major RGBA f32 with pixel (0, 0) top-left; the beauty alpha is 1
+ * exactly once per pixel per sample; depth is the camera-plane z, +inf on
+ * a miss. Colors are ACEScg.
+ *
+ * Errors: every call returns a cn_status. CN_OK is 0; on any other value
+ * cn_last_error() returns a thread-local message. The reference panics on
+ * programmer bugs (wavefront.rs:493-498, :618-623; gpu/submit.rs:265-269);
+ * this ABI returns CN_E_INVALID instead.
+ *
+ * Ownership: the caller owns every host array and may free it as soon as
+ * the call that received it returns (the library copies). The library owns
+ * device memory. Destroy films, renderers and scenes before their context
+ * (the order crates/cenote/src/gpu/mod.rs:224-239 keeps).
+ *
+ * Threading: one context per process; a renderer or film is used by one
+ * thread at a time. Calls are synchronous unless the name ends in _async,
+ * in which case they are ordered on the renderer's stream.
+ */
+#ifndef CENOTE_HIP_H
+#define CENOTE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the mapping of crates/cenote/src/error.rs:8-76. */
+typedef enum cn_status {
+    CN_OK = 0,
+    CN_E_NO_DEVICE = 1,    /* Error::NoCapableGpu */
+    CN_E_ALLOC = 2,        /* Error::Allocation */
+    CN_E_SCENE = 3,        /* Error::Scene: a rejected scene description */
+    CN_E_INVALID = 4,      /* a programmer bug the reference panics on */
+    CN_E_DEVICE_FAULT = 5  /* Error::Vulkan: the device reported a failure */
+} cn_status;
+
+/* LightSampling (wavefront.rs:428-439, shaders/lights.slang:32-34). */
+enum {
+    CN_LIGHT_SAMPLING_MIS = 0,
+    CN_LIGHT_SAMPLING_BSDF = 1,
+    CN_LIGHT_SAMPLING_NEE = 2
+};
+
+/* Delta-light kinds (lights.rs:57-73). */
+enum {
+    CN_LIGHT_DISTANT = 1,
+    CN_LIGHT_POINT = 2
+};
+
+/* "No texture" in a material's texture slots (material.rs:23). */
+#define CN_TEXTURE_NONE 0xffffffffu
+
+/* Wavefront::DEFAULT_CAPACITY and DEFAULT_MAX_BOUNCES (wavefront.rs:462-471). */
+#define CN_DEFAULT_CAPACITY (1u << 20)
+#define CN_DEFAULT_MAX_BOUNCES 8u
+
+/*
+ * One instance's lowered OpenPBR surface, 144 bytes, field for field the
+ * GPU record of crates/cenote/src/material.rs:28-116 (colors in ACEScg,
+ * weights already clamped by lowering, scene/lower.rs:316-399). Texture
+ * slots must be CN_TEXTURE_NONE: material textures are outside this path.
+ */
+typedef struct cn_material {
+    float base_color[3];
+    float base_roughness;
+    float emission[3];
+    float metalness;
+    float specular_weight;
+    float specular_roughness;
+    float specular_ior;
+    float transmission_weight;
+    float transmission_color[3];
+    float transmission_depth;
+    float coat_color[3];
+    float coat_weight;
+    float coat_roughness;
+    float coat_ior;
+    float coat_darkening;
+    float fuzz_weight;
+    float fuzz_color[3];
+    float fuzz_roughness;
+    float opacity;
+    uint32_t thin_walled;
+    uint32_t base_color_texture;
+    uint32_t specular_roughness_texture;
+    uint32_t metalness_texture;
+    uint32_t emission_texture;
+    uint32_t opacity_texture;
+    uint32_t normal_texture;
+} cn_material;
+
+/*
+ * A triangle mesh in object space (scene/mod.rs:58-76): packed xyz
+ * positions, one unit shading normal per vertex, one uv per vertex (may be
+ * NULL: zeros), and counter-clockwise index triples.
+ */
+typedef struct cn_mesh {
+    const float* positions;
+    const float* normals;
+    const float* uvs;
+    const uint32_t* indices;
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+} cn_mesh;
+
+/*
+ * One placed object (scene/mod.rs:78-87 `Object`, plus the
+ * camera-visibility flag of scene/mod.rs:394-400): the mesh it places, the
+ * rows of its 3x4 object-to-world transform (must be invertible), and its
+ * material. The instance's position in the array is its custom index.
+ */
+typedef struct cn_instance {
+    uint32_t mesh;
+    uint32_t camera_visible;
+    float object_to_world[12];
+    cn_material material;
+} cn_instance;
+
+/* A delta light (lights.rs:57-73): distant `vector` = unit travel
+ * direction and `color` = irradiance; point `vector` = position and
+ * `color` = intensity. ACEScg. */
+typedef struct cn_delta_light {
+    uint32_t kind;
+    float vector[3];
+    float color[3];
+} cn_delta_light;
+
+/* An equirect environment, RGBA f32 ACEScg texels, row 0 = +Y
+ * (environment.rs:33-79). */
+typedef struct cn_environment {
+    uint32_t width;
+    uint32_t height;
+    const float* texels;
+} cn_environment;
+
+/* The camera (scene/mod.rs:397-447, with the lens resolved the way
+ * scene/lower.rs:219-234 does): aperture_radius 0 is a pinhole;
+ * focus_distance <= 0 focuses at look_at. */
+typedef struct cn_camera {
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float vfov_degrees;
+    float aperture_radius;
+    float focus_distance;
+} cn_camera;
+
+typedef struct cn_scene_desc {
+    const cn_mesh* meshes;
+    uint32_t mesh_count;
+    const cn_instance* instances;
+    uint32_t instance_count;
+    const cn_delta_light* delta_lights;
+    uint32_t delta_light_count;
+    cn_environment environment;
+    cn_camera camera;
+} cn_scene_desc;
+
+/* The 80-byte light record as the kernels read it (lights.rs:81-115). */
+typedef struct cn_light_record {
+    float anchor[3];
+    float pdf;
+    float edge1[3];
+    float alias_threshold;
+    float edge2[3];
+    uint32_t alias_index;
+    float emission[3];
+    uint32_t instance;
+    uint32_t kind;
+    uint32_t primitive;
+    uint32_t pad[2];
+} cn_light_record;
+
+/* Scene facts for tests and measurement. */
+typedef struct cn_scene_info {
+    uint32_t instance_count;
+    uint32_t triangle_count;   /* flattened world triangles */
+    uint32_t light_count;      /* records in the alias table */
+    uint32_t env_width;
+    uint32_t env_height;
+    float env_select_prob;     /* scene/mod.rs:665-671 */
+    double env_power;          /* environment.rs:203-210 */
+    uint32_t bvh_node_count;
+    uint32_t bvh_width;        /* children per node */
+    uint32_t bvh_node_bytes;
+    uint32_t bvh_triangle_bytes;
+    uint32_t bvh_max_depth;
+    uint32_t pad;
+    double bvh_build_seconds;
+    double prep_seconds;
+} cn_scene_info;
+
+/* Per-bounce queue counts of the most recent wave (the audit of
+ * wavefront.rs:874-913): rays traced, hits, misses, shadow rays. */
+typedef struct cn_wave_counts {
+    uint32_t bounces;
+    uint32_t rays[256];
+    uint32_t hits[256];
+    uint32_t misses[256];
+    uint32_t shadows[256];
+} cn_wave_counts;
+
+/* BVH traversal work of the most recent wave traced with counters on. */
+typedef struct cn_traversal_counts {
+    uint64_t ray_nodes;      /* nodes fetched by closest-hit rays */
+    uint64_t ray_triangles;  /* triangles tested by closest-hit rays */
+    uint64_t shadow_nodes;
+    uint64_t shadow_triangles;
+} cn_traversal_counts;
+
+/* Device time per stage kernel, summed over the launches since the last
+ * read (HIP events recorded on the renderer's stream around each launch). */
+typedef struct cn_kernel_times {
+    uint32_t kernels;
+    char names[8][24];
+    double total_ms[8];
+    uint64_t launches[8];
+} cn_kernel_times;
+
+typedef struct cn_context cn_context;
+typedef struct cn_scene cn_scene;
+typedef struct cn_renderer cn_renderer;
+typedef struct cn_film cn_film;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* cn_last_error(void);
+
+/* Context::new (crates/cenote/src/gpu/mod.rs:93). */
+int cn_context_create(int device, cn_context** out);
+int cn_context_destroy(cn_context* ctx);
+/* Device name and compute-unit count, for reports. */
+int cn_context_describe(cn_context* ctx, char* name, size_t name_capacity,
+                        uint32_t* compute_units);
+
+/* Scene::new (scene/mod.rs:254-338): copies the description, lowers the
+ * light list (lights.rs:175-224) and environment tables
+ * (environment.rs:130-218) on the host, builds the BVH that replaces the
+ * TLAS/BLAS pair (gpu/accel.rs:82-163), and uploads everything. */
+int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out);
+int cn_scene_destroy(cn_scene* scene);
+/* Camera edits between frames (Scene::camera_mut, scene/mod.rs:391-394). */
+int cn_scene_set_camera(cn_scene* scene, const cn_camera* camera);
+int cn_scene_info_get(cn_scene* scene, cn_scene_info* info);
+/* Host copies of the bit-exact tables, for parity checks. */
+int cn_scene_light_records(cn_scene* scene, cn_light_record* out, uint32_t capacity,
+                           uint32_t* count);
+int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional,
+                        float* pdfs);
+
+/* Wavefront::new / Renderer::with_max_bounces (wavefront.rs:486-540,
+ * render/mod.rs:151): capacity paths in the pool (0 = default),
+ * max_bounces in 1..255, light_sampling a CN_LIGHT_SAMPLING_* value. */
+int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
+                       uint32_t light_sampling, cn_renderer** out);
+int cn_renderer_destroy(cn_renderer* renderer);
+/* Count BVH node and triangle fetches in later waves (slower). */
+int cn_renderer_set_counters(cn_renderer* renderer, int enabled);
+int cn_renderer_wave_counts(cn_renderer* renderer, cn_wave_counts* out);
+int cn_renderer_traversal_counts(cn_renderer* renderer, cn_traversal_counts* out);
+/* Time every stage launch with HIP events (measurement only; the
+ * reference has no counterpart — it carries no GPU timestamps). */
+int cn_renderer_set_profiling(cn_renderer* renderer, int enabled);
+/* Per-kernel device time since the last call; resets the totals. */
+int cn_renderer_kernel_times(cn_renderer* renderer, cn_kernel_times* out);
+
+/* Wavefront::trace (wavefront.rs:571-582): one sample of a width x height
+ * target into `radiance` (host RGBA f32, width*height*4 floats). */
+int cn_trace(cn_renderer* renderer, cn_scene* scene, float* radiance, uint32_t width,
+             uint32_t height, uint32_t sample);
+/* As cn_trace, also returning the wave's AOV accumulators (any may be
+ * NULL): albedo and normal RGBA f32, depth f32. */
+int cn_trace_aov(cn_renderer* renderer, cn_scene* scene, float* radiance, float* albedo,
+                 float* normal, float* depth, uint32_t width, uint32_t height,
+                 uint32_t sample);
+/* The bounce-0 hits of one wave (the first-hit identity check): instance
+ * and primitive per pixel (0xffffffff on a miss) and barycentrics. */
+int cn_first_hits(cn_renderer* renderer, cn_scene* scene, uint32_t width, uint32_t height,
+                  uint32_t sample, uint32_t* instance, uint32_t* primitive, float* bary);
+
+/* Film (render/film.rs:66-210). */
+int cn_film_create(cn_context* ctx, uint32_t width, uint32_t height, cn_film** out);
+int cn_film_destroy(cn_film* film);
+/* Film::reset: the next accumulate overwrites (sample index 0). */
+int cn_film_reset(cn_film* film);
+int cn_film_samples(cn_film* film, uint32_t* samples);
+/* Renderer::accumulate (render/mod.rs:245-259): one wave at sample index
+ * film.samples, folded into the film's sums; film.samples += 1. */
+int cn_accumulate(cn_renderer* renderer, cn_scene* scene, cn_film* film);
+/* `count` consecutive accumulates, stream-ordered, no host sync between
+ * them; the sums are bitwise those of `count` cn_accumulate calls. */
+int cn_accumulate_async(cn_renderer* renderer, cn_scene* scene, cn_film* film,
+                        uint32_t count);
+/* Block until the renderer's stream has drained. */
+int cn_renderer_synchronize(cn_renderer* renderer);
+/* Film::averages (render/film.rs:183-197): sums / samples on the host.
+ * Any output may be NULL. */
+int cn_film_averages(cn_film* film, float* beauty, float* albedo, float* normal,
+                     float* depth);
+/* The raw f32 sums (render/tests.rs reads these). Any output may be NULL. */
+int cn_film_sums(cn_film* film, float* beauty, float* albedo, float* normal, float* depth);
+/* Replace the film's sums (tests of the non-finite guard, render/tests.rs:1465-1520). */
+int cn_film_set_sample(cn_film* film, const float* beauty, const float* albedo,
+                       const float* normal, const float* depth);
+/* Fold the film's current sample buffers into its sums (the accumulate
+ * kernel alone, shaders/accumulate.slang:46-67); film.samples += 1. */
+int cn_film_accumulate_sample(cn_renderer* renderer, cn_film* film);
+/* Renderer::resolve (render/mod.rs:278-299): sums / samples on the device,
+ * copied to the host outputs (any may be NULL). */
+int cn_resolve(cn_renderer* renderer, cn_film* film, float* beauty, float* albedo,
+               float* normal, float* depth);
+
+/* The rng_test kernel (shaders/rng_test.slang:21-29): the first `count`
+ * sample_2d points and sample_1d values of key (pixel, dimension),
+ * computed on the device. points2d has 2*count floats. */
+int cn_sample_sequence(cn_context* ctx, uint32_t pixel, uint32_t dimension, uint32_t count,
+                       float* points2d, float* values1d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CENOTE_HIP_H */

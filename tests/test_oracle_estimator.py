@@ -1,0 +1,149 @@
+"""Pins for the oracle's estimator (the per-path restatement of
+shaders/raygen.slang .. trace_shadow.slang): the reference's closed-form
+tests of crates/cenote/src/render/tests.rs and wavefront.rs, at sizes the
+CPU finishes in seconds."""
+
+import numpy as np
+import pytest
+
+from cenote_amd import workloads
+from cenote_amd.scene import Material, OpenPBR, acescg_from_rec709
+from oracle import oracle as orc
+
+from reference_scenes import agreement_scene, aov_quad_scene, distant_light_scene, point_light_scene
+
+BSDF_ONLY, NEE_ONLY, MIS = 1, 2, 0
+
+
+def test_lambert_furnace_closes_per_sample():
+    """render/tests.rs:185-203: BSDF-only, every sample = albedo x sky."""
+    scene = orc.OracleScene(workloads.furnace(Material.matte((1.0, 1.0, 1.0), 0.0)))
+    for sample in range(4):
+        img = scene.trace(32, 32, sample, light_sampling=BSDF_ONLY)
+        assert np.allclose(img[..., :3], 0.5, atol=1e-3)
+        assert np.all(img[..., 3] == 1.0)
+
+
+def test_rough_diffuse_furnace_mean():
+    """render/tests.rs:205-213: EON at roughness 1 closes to 0.005."""
+    scene = orc.OracleScene(workloads.furnace(Material.matte((1.0, 1.0, 1.0), 1.0)))
+    b, _, _, _ = scene.accumulate(32, 32, 0, 64)
+    assert abs(b[..., 0].mean() / 64 - 0.5) < 0.005
+
+
+def test_ray_offsets_hold_at_scene_scale():
+    """render/tests.rs:226-253: the plane 1e4 m out and 1000x larger."""
+    data = workloads.furnace(Material.matte((0.5, 0.5, 0.5), 0.0), center=(1e4, 0.0, 1e4), scale=1e3)
+    scene = orc.OracleScene(data)
+    for sample in range(4):
+        img = scene.trace(32, 32, sample, light_sampling=BSDF_ONLY)
+        assert np.allclose(img[..., :3], 0.25, atol=1e-3)
+
+
+def test_the_furnace_closes_through_a_thin_lens():
+    """render/tests.rs:265-286"""
+    data = workloads.furnace(Material.matte((0.5, 0.5, 0.5), 0.0), aperture=0.05, focus=1.5)
+    scene = orc.OracleScene(data)
+    for sample in range(4):
+        img = scene.trace(32, 32, sample, light_sampling=BSDF_ONLY)
+        assert np.allclose(img[..., :3], 0.25, atol=1e-3)
+
+
+def test_a_distant_light_is_analytically_exact():
+    """render/tests.rs:352-374"""
+    img = orc.OracleScene(distant_light_scene()).trace(16, 16, 0)
+    assert np.allclose(img[..., :3], 1.0, atol=2e-3)
+
+
+def test_a_point_light_is_analytically_exact():
+    """render/tests.rs:383-408"""
+    img = orc.OracleScene(point_light_scene()).trace(16, 16, 0)
+    assert np.allclose(img[..., :3], 1.0, atol=5e-3)
+
+
+WHITE = (1.0, 1.0, 1.0)
+FURNACE_MATRIX = {
+    "metal": Material.metal(WHITE, 0.5),
+    "smooth_metal": Material.metal(WHITE, 0.05),
+    "glossy": Material.glossy(WHITE, 0.0, 0.3),
+    "ior_2": OpenPBR(base_color=WHITE, specular_ior=2.0, specular_roughness=0.5).lower(),
+    "coat": OpenPBR(base_color=WHITE, coat_weight=1.0, coat_roughness=0.2).lower(),
+    "fuzz": OpenPBR(base_color=WHITE, fuzz_weight=1.0).lower(),
+    "full_stack": OpenPBR(base_color=WHITE, base_metalness=0.5, coat_weight=0.7, fuzz_weight=0.5,
+                          specular_roughness=0.4).lower(),
+    "glass": Material.glass(0.3, 1.5),
+    "thin_glass": OpenPBR(transmission_weight=1.0, geometry_thin_walled=True, specular_roughness=0.2).lower(),
+    "half_opacity": OpenPBR(base_color=WHITE, specular_weight=0.0, geometry_opacity=0.5).lower(),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FURNACE_MATRIX))
+def test_openpbr_furnace_matrix(name):
+    """render/tests.rs:422-488: a white material of any construction returns
+    the sky, within the reference's 1.5%."""
+    scene = orc.OracleScene(workloads.furnace(FURNACE_MATRIX[name]))
+    b, _, _, _ = scene.accumulate(24, 24, 0, 64, max_bounces=64)
+    mean = b[..., :3].mean() / 64
+    assert abs(mean - 0.5) / 0.5 < 0.015, (name, mean)
+
+
+def test_aovs_record_albedo_normal_and_depth():
+    """render/tests.rs:1117-1221"""
+    scene = orc.OracleScene(aov_quad_scene())
+    b, a, n, d = scene.accumulate(64, 64, 0, 4)
+    a, n, d = a / 4, n / 4, d / 4
+    expected = np.array(acescg_from_rec709((0.7, 0.2, 0.1)), np.float32)
+    for x, y in [(32, 32), (44, 32), (32, 22)]:
+        assert np.allclose(a[y, x, :3], expected, atol=1e-5)
+        assert np.allclose(n[y, x, :3], [0.0, 0.0, 1.0], atol=1e-5)
+        assert abs(d[y, x] - 2.0) < 1e-3
+    assert np.allclose(a[2, 2, :3], 1.0, atol=1e-5)
+    assert np.allclose(n[2, 2, :3], 0.0, atol=1e-5)
+    assert np.isposinf(d[2, 2])
+
+
+def test_light_sampling_strategies_agree():
+    """wavefront.rs:1246-1557: MIS, BSDF-only and NEE-only converge to the
+    same image (here: the same mean over the frame, within 3%)."""
+    scene = orc.OracleScene(agreement_scene())
+    means = {}
+    for mode in (MIS, BSDF_ONLY, NEE_ONLY):
+        b, _, _, _ = scene.accumulate(32, 32, 0, 256, light_sampling=mode)
+        means[mode] = b[..., :3].mean() / 256
+    assert abs(means[BSDF_ONLY] - means[MIS]) / means[MIS] < 0.03, means
+    assert abs(means[NEE_ONLY] - means[MIS]) / means[MIS] < 0.03, means
+
+
+def test_threads_do_not_change_the_image():
+    scene = orc.OracleScene(workloads.cornell())
+    a = scene.trace(64, 64, 3, threads=1)
+    b = scene.trace(64, 64, 3, threads=7)
+    assert np.array_equal(a, b)
+
+
+def test_every_pixel_finishes_exactly_once():
+    scene = orc.OracleScene(workloads.cornell())
+    img = scene.trace(64, 64, 0)
+    assert np.all(img[..., 3] == 1.0)
+
+
+def test_camera_basis_is_the_glam_basis():
+    """scene/mod.rs:448-461: forward unit, right = fwd x up scaled by
+    tan(vfov/2) * aspect, up = right x fwd scaled by tan(vfov/2)."""
+    import ctypes as C
+
+    from cenote_amd._abi import Camera, f3
+
+    cam = Camera(f3(0, 5.5, 11), f3(0, 0, 0), f3(0, 1, 0), 40.0, 0.0, 0.0)
+    out = (C.c_float * 9)()
+    orc.lib().orc_camera_basis(C.cast(C.pointer(cam), C.c_void_p), 16 / 9, out)
+    right, up, fwd = np.array(out[0:3]), np.array(out[3:6]), np.array(out[6:9])
+    f = np.array([0, -5.5, -11.0])
+    f /= np.linalg.norm(f)
+    r = np.cross(f, [0, 1, 0])
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    t = np.tan(np.radians(20.0))
+    assert np.allclose(fwd, f, atol=1e-6)
+    assert np.allclose(right, r * t * 16 / 9, atol=1e-6)
+    assert np.allclose(up, u * t, atol=1e-6)
