@@ -181,6 +181,12 @@ def main():
     kernel_ms = {k: round(v[0] / steps, 4) for k, v in times.items()}
     kernel_gbs = {k: round(per_kernel_bytes[k] / (times[k][0] * 1e-3) / 1e9, 1)
                   for k in times if times[k][0] > 0 and k in per_kernel_bytes}
+    log(f"per wave: rays {totals['rays'] / steps:.0f}, hits {totals['hits'] / steps:.0f}, "
+        f"misses {totals['misses'] / steps:.0f}, shadows {totals['shadows'] / steps:.0f}; "
+        f"nodes/ray {traversal['ray_nodes'] / max(1, totals['rays']):.1f}, "
+        f"tris/ray {traversal['ray_triangles'] / max(1, totals['rays']):.1f}, "
+        f"nodes/shadow {traversal['shadow_nodes'] / max(1, totals['shadows']):.1f}, "
+        f"tris/shadow {traversal['shadow_triangles'] / max(1, totals['shadows']):.1f}")
     log(f"timed {steps} waves in {elapsed:.3f} s: {value:.1f} Mpaths/s, {grays:.3f} Grays/s")
     log(f"per-wave kernel ms: {kernel_ms}")
     log(f"per-kernel algorithmic GB/s: {kernel_gbs}")

@@ -21,12 +21,12 @@
 #include "cn_math.h"
 #include "cn_records.h"
 #include "cn_shading.h"
+#include "cn_traverse.h"
 
 namespace cn {
 
 constexpr uint32_t BLOCK = 256;
 constexpr uint32_t ROULETTE_START = 3;   // shade_surface.slang:53
-constexpr uint32_t TRANSPARENT_CAP = 8;  // intersect.slang:23
 
 // --- wave64 queue aggregation ------------------------------------------------
 
@@ -59,259 +59,6 @@ __device__ __forceinline__ void add_pixel(float4* target, uint32_t pixel, v3 c, 
     target[pixel] = v;
 }
 
-// --- traversal ---------------------------------------------------------------
-
-// The traversal stack: the first STACK_LDS entries of each lane live in LDS
-// (entry k of thread t at [k * BLOCK + t], bank-conflict free), the rest in
-// a per-lane spill area in HBM laid out [entry][lane] so spills coalesce.
-struct Stack {
-    uint32_t* lds;
-    uint32_t* spill;
-    uint32_t lanes;
-    uint32_t lane;
-    uint32_t sp;
-    // The builder rejects trees that could need more than STACK_MAX entries;
-    // a push past it is dropped all the same, so every pop stays in bounds.
-    __device__ __forceinline__ void push(uint32_t v) {
-        if (sp >= STACK_MAX) return;
-        if (sp < STACK_LDS) lds[sp * BLOCK + threadIdx.x] = v;
-        else spill[(sp - STACK_LDS) * lanes + lane] = v;
-        ++sp;
-    }
-    __device__ __forceinline__ uint32_t pop() {
-        --sp;
-        return sp < STACK_LDS ? lds[sp * BLOCK + threadIdx.x] : spill[(sp - STACK_LDS) * lanes + lane];
-    }
-};
-
-__device__ __forceinline__ float safe_rcp(float x) {
-    return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x);
-}
-
-__device__ __forceinline__ float pick(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
-
-// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2(1), 2013),
-// restated identically by the CPU oracle so both agree on the hit bit for
-// bit: t in [0, tmax], barycentrics (b1, b2) on v1 and v2.
-struct WRay {
-    v3 o;
-    int kx, ky, kz;
-    float Sx, Sy, Sz;
-};
-
-__device__ __forceinline__ WRay make_wray(v3 o, v3 d) {
-    WRay w;
-    w.o = o;
-    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int kz = 0;
-    float m = ax;
-    if (ay > m) { kz = 1; m = ay; }
-    if (az > m) { kz = 2; }
-    int kx = kz == 2 ? 0 : kz + 1;
-    int ky = kx == 2 ? 0 : kx + 1;
-    const float dz = pick(d, kz);
-    if (dz < 0.0f) { const int t = kx; kx = ky; ky = t; }
-    w.kx = kx;
-    w.ky = ky;
-    w.kz = kz;
-    w.Sx = pick(d, kx) / dz;
-    w.Sy = pick(d, ky) / dz;
-    w.Sz = 1.0f / dz;
-    return w;
-}
-
-__device__ __forceinline__ bool hit_triangle(const WRay& w, v3 v0, v3 v1, v3 v2, float tmax, float* t_out,
-                                             float* b1, float* b2) {
-    const v3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
-    float Az = pick(A, w.kz), Bz = pick(B, w.kz), Cz = pick(C, w.kz);
-    const float Ax = pick(A, w.kx) - w.Sx * Az, Ay = pick(A, w.ky) - w.Sy * Az;
-    const float Bx = pick(B, w.kx) - w.Sx * Bz, By = pick(B, w.ky) - w.Sy * Bz;
-    const float Cx = pick(C, w.kx) - w.Sx * Cz, Cy = pick(C, w.ky) - w.Sy * Cz;
-    float U = Cx * By - Cy * Bx;
-    float Vv = Ax * Cy - Ay * Cx;
-    float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || Vv == 0.0f || W == 0.0f) {
-        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-        Vv = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
-    }
-    if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
-    const float det = U + Vv + W;
-    if (det == 0.0f) return false;
-    Az = w.Sz * Az;
-    Bz = w.Sz * Bz;
-    Cz = w.Sz * Cz;
-    const float T = U * Az + Vv * Bz + W * Cz;
-    const float rcp = 1.0f / det;
-    const float t = T * rcp;
-    if (!(t >= 0.0f) || !(t <= tmax)) return false;
-    *t_out = t;
-    *b1 = Vv * rcp;
-    *b2 = W * rcp;
-    return true;
-}
-
-struct TraceHit {
-    bool hit;
-    float t;
-    uint32_t inst, prim;
-    float b1, b2;
-};
-
-enum class TraceKind { Closest, Shadow };
-
-struct ShadowQuery {
-    uint32_t target_inst, target_prim;
-    float t_early;      // an opaque non-target hit nearer than this occludes outright
-    bool occluded;      // set when such a hit, or a non-target closest hit, was found
-    float attenuation;  // product over fractional-opacity crossings in front of the end
-};
-
-struct OpacityQuery {
-    uint32_t pixel, sample, bounce, crossings;
-};
-
-// Closest hit over the BVH4 within [0, tmax] (intersect.slang:64-88,
-// trace_shadow.slang:50-72 — the driver traversal both call, rebuilt).
-template <TraceKind KIND>
-__device__ __forceinline__ TraceHit traverse(const SceneView& s, v3 o, v3 d, float tmax, uint32_t mask, Stack& st,
-                                             OpacityQuery* oq, ShadowQuery* sq, uint32_t* nodes_visited,
-                                             uint32_t* tris_tested) {
-    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-    const WRay w = make_wray(o, d);
-    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
-    // Fractional-opacity crossings seen by a shadow ray (rare: only scenes
-    // with opacity < 1), kept to multiply in increasing t at the end.
-    float ct[16], ctr[16];
-    int ncross = 0;
-    float overflow = 1.0f;
-    st.sp = 0;
-    uint32_t cur = 0;
-    uint32_t nv = 0, nt = 0;
-    while (true) {
-        if (!(cur & CHILD_LEAF)) {
-            const BvhNode4 n = s.nodes[cur];
-            ++nv;
-            const float lx[4] = {n.lo_x.x, n.lo_x.y, n.lo_x.z, n.lo_x.w};
-            const float hx[4] = {n.hi_x.x, n.hi_x.y, n.hi_x.z, n.hi_x.w};
-            const float ly[4] = {n.lo_y.x, n.lo_y.y, n.lo_y.z, n.lo_y.w};
-            const float hy[4] = {n.hi_y.x, n.hi_y.y, n.hi_y.z, n.hi_y.w};
-            const float lz[4] = {n.lo_z.x, n.lo_z.y, n.lo_z.z, n.lo_z.w};
-            const float hz[4] = {n.hi_z.x, n.hi_z.y, n.hi_z.z, n.hi_z.w};
-            const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-            uint32_t hitc[4];
-            float hitd[4];
-            int nh = 0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float ax = fmaf(lx[c], ix, -ox), bx = fmaf(hx[c], ix, -ox);
-                const float ay = fmaf(ly[c], iy, -oy), by = fmaf(hy[c], iy, -oy);
-                const float az = fmaf(lz[c], iz, -oz), bz = fmaf(hz[c], iz, -oz);
-                const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-                const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), best.t)) * 1.0000004f;
-                if (t0 <= t1 && ch[c] != CHILD_EMPTY) {
-                    // insertion into the near-to-far list
-                    int k = nh++;
-                    while (k > 0 && hitd[k - 1] > t0) {
-                        hitd[k] = hitd[k - 1];
-                        hitc[k] = hitc[k - 1];
-                        --k;
-                    }
-                    hitd[k] = t0;
-                    hitc[k] = ch[c];
-                }
-            }
-            if (nh == 0) {
-                if (st.sp == 0) break;
-                cur = st.pop();
-                continue;
-            }
-            for (int k = nh - 1; k > 0; --k) st.push(hitc[k]);
-            cur = hitc[0];
-            continue;
-        }
-        // leaf
-        const uint32_t first = cur & LEAF_FIRST_MASK;
-        const uint32_t count = ((cur >> 27) & 15u) + 1u;
-        for (uint32_t k = 0; k < count; ++k) {
-            const BvhTriangle tri = s.tris[first + k];
-            ++nt;
-            const uint32_t flags = __float_as_uint(tri.v2.w);
-            if (((flags >> 8) & mask) == 0) continue;
-            float t, b1, b2;
-            if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), best.t, &t, &b1, &b2)) continue;
-            const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
-            if (best.hit && t == best.t && (inst > best.inst || (inst == best.inst && prim > best.prim))) continue;
-            if (KIND == TraceKind::Closest) {
-                if (flags & TRI_NON_OPAQUE) {
-                    const float opacity = s.materials[inst].opacity;
-                    const bool commit = oq->crossings >= TRANSPARENT_CAP ||
-                                        hash_uniform(oq->pixel, oq->sample, oq->bounce, oq->crossings) < opacity;
-                    ++oq->crossings;
-                    if (!commit) continue;
-                }
-            } else {
-                const bool target = inst == sq->target_inst && prim == sq->target_prim;
-                if (!target) {
-                    if (flags & TRI_NON_OPAQUE) {
-                        const float tr = 1.0f - s.materials[inst].opacity;
-                        if (ncross < 16) {
-                            ct[ncross] = t;
-                            ctr[ncross] = tr;
-                            ++ncross;
-                        } else {
-                            overflow *= tr;
-                        }
-                        continue;
-                    }
-                    if (t < sq->t_early) {
-                        sq->occluded = true;
-                        *nodes_visited += nv;
-                        *tris_tested += nt;
-                        return best;
-                    }
-                }
-            }
-            best.hit = true;
-            best.t = t;
-            best.inst = inst;
-            best.prim = prim;
-            best.b1 = b1;
-            best.b2 = b2;
-        }
-        if (st.sp == 0) break;
-        cur = st.pop();
-    }
-    *nodes_visited += nv;
-    *tris_tested += nt;
-    if (KIND == TraceKind::Shadow) {
-        const bool target_hit = best.hit && best.inst == sq->target_inst && best.prim == sq->target_prim;
-        if (best.hit && !target_hit) {
-            sq->occluded = true;
-            return best;
-        }
-        const float tEnd = best.hit ? best.t : tmax;
-        // insertion sort by (t, transparency), then multiply near to far
-        for (int i = 1; i < ncross; ++i) {
-            const float kt = ct[i], kr = ctr[i];
-            int j = i - 1;
-            while (j >= 0 && (ct[j] > kt || (ct[j] == kt && ctr[j] > kr))) {
-                ct[j + 1] = ct[j];
-                ctr[j + 1] = ctr[j];
-                --j;
-            }
-            ct[j + 1] = kt;
-            ctr[j + 1] = kr;
-        }
-        float att = 1.0f;
-        for (int i = 0; i < ncross; ++i) {
-            if (ct[i] < tEnd) att *= ctr[i];
-        }
-        sq->attenuation = att * overflow;
-    }
-    return best;
-}
 
 // --- raygen (raygen.slang:64-112) --------------------------------------------------
 
@@ -382,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(CameraView cam, PathPool p, Wa
 
 // --- intersect (intersect.slang:45-88) ----------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_intersect(SceneView s, PathPool p, uint32_t* counters, uint32_t base,
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_intersect(SceneView s, PathPool p, uint32_t* counters, uint32_t base,
                                                      uint32_t range_count, uint32_t bounce, uint32_t sample,
                                                      int counting) {
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
@@ -401,8 +148,7 @@ __global__ __launch_bounds__(BLOCK) void k_intersect(SceneView s, PathPool p, ui
             path = bounce == 0 ? i : rays[i];
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
-            OpacityQuery oq{base + path, sample, bounce, 0};
-            const TraceHit h = traverse<TraceKind::Closest>(s, o, d, 1e30f, mask, st, &oq, nullptr, &nodes, &tris);
+            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, base + path, sample, bounce, nodes, tris);
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
         }
@@ -728,18 +474,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace_shadow(SceneView s, PathPool p,
         const float4 o4 = p.shadow_origin[i];
         const float4 d4 = p.shadow_dir[i];
         const float4 c4 = p.shadow_contrib[i];
-        ShadowQuery sq;
-        sq.target_inst = __float_as_uint(c4.w);
-        sq.target_prim = p.shadow_prim[i];
+        const uint32_t target = __float_as_uint(c4.w);
         // Any solid non-target hit short of the light's own distance
-        // (tmax / 1.001 for a triangle target) must lie in front of the
-        // target, so it occludes without finding the closest hit.
-        sq.t_early = sq.target_inst == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
-        sq.occluded = false;
-        sq.attenuation = 1.0f;
-        traverse<TraceKind::Shadow>(s, xyz(o4), xyz(d4), o4.w, MASK_ALL, st, nullptr, &sq, &nodes, &tris);
-        if (sq.occluded || sq.attenuation < 1e-4f) continue;
-        add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * sq.attenuation, 0.0f);
+        // (tmax / 1.001 for a triangle target) lies in front of the target,
+        // so it occludes without finding the closest hit.
+        const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
+        const float att = trace_visibility(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[i], t_early, st, nodes,
+                                           tris);
+        if (att < 1e-4f) continue;
+        add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
     }
     if (counting) {
         wave_add_u64(&p.traversal_counts[2], nodes);
@@ -808,8 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_hits(SceneView s, CameraView ca
         v3 o, d;
         float cf;
         camera_ray(cam, width, height, pixel, sample, &o, &d, &cf);
-        OpacityQuery oq{pixel, sample, 0, 0};
-        const TraceHit h = traverse<TraceKind::Closest>(s, o, d, 1e30f, MASK_CAMERA, st, &oq, nullptr, &nodes, &tris);
+        const TraceHit h = trace_closest(s, o, d, 1e30f, MASK_CAMERA, st, pixel, sample, 0, nodes, tris);
         inst_out[pixel] = h.hit ? h.inst : 0xffffffffu;
         prim_out[pixel] = h.hit ? h.prim : 0xffffffffu;
         bary_out[2 * pixel] = h.hit ? h.b1 : 0.0f;

@@ -1,0 +1,441 @@
+// cn_traverse.h — BVH4 traversal for the intersect and trace_shadow stages
+// (the driver ray-query loops of shaders/intersect.slang:64-88 and
+// trace_shadow.slang:50-72, rebuilt for CDNA, which has no ray tracing
+// units).
+//
+// One lane per ray, closest-first descent with a short stack (LDS, spilling
+// to HBM), and the speculative while-while schedule of Aila & Laine
+// ("Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009): a
+// lane that reaches a leaf postpones it and keeps descending until every
+// lane of the wave64 holds a leaf, so leaf tests run with full SIMD lanes.
+#pragma once
+
+#include "cn_launch.h"
+#include "cn_math.h"
+#include "cn_records.h"
+
+namespace cn {
+
+constexpr uint32_t TRAVERSAL_BLOCK = 256;
+constexpr uint32_t TRANSPARENT_CAP = 8;     // intersect.slang:23
+constexpr uint32_t ATTENUATION_CROSSINGS = 16;
+constexpr uint32_t NODE_DONE = CHILD_EMPTY;  // never pushed: empty slots fail every box test
+
+// The traversal stack: the first STACK_LDS entries of each lane live in LDS
+// (entry k of thread t at [k * TRAVERSAL_BLOCK + t], bank-conflict free),
+// the rest in a per-lane spill area in HBM laid out [entry][lane] so spills
+// coalesce. The builder rejects trees that could need more than STACK_MAX
+// entries; a push past it is dropped all the same, so every pop stays in
+// bounds.
+struct Stack {
+    uint32_t* lds;
+    uint32_t* spill;
+    uint32_t lanes;
+    uint32_t lane;
+    uint32_t sp;
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (sp >= STACK_MAX) return;
+        if (sp < STACK_LDS) lds[sp * TRAVERSAL_BLOCK + threadIdx.x] = v;
+        else spill[(sp - STACK_LDS) * lanes + lane] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop_or_done() {
+        if (sp == 0) return NODE_DONE;
+        --sp;
+        return sp < STACK_LDS ? lds[sp * TRAVERSAL_BLOCK + threadIdx.x] : spill[(sp - STACK_LDS) * lanes + lane];
+    }
+};
+
+__device__ __forceinline__ float safe_rcp(float x) {
+    return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x);
+}
+
+__device__ __forceinline__ float pick(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2(1), 2013),
+// restated identically by the CPU oracle so both agree on the hit bit for
+// bit: t in [0, tmax], barycentrics (b1, b2) on v1 and v2.
+struct WRay {
+    v3 o;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WRay make_wray(v3 o, v3 d) {
+    WRay w;
+    w.o = o;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) {
+        kz = 1;
+        m = ay;
+    }
+    if (az > m) kz = 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dz = pick(d, kz);
+    if (dz < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.Sx = pick(d, kx) / dz;
+    w.Sy = pick(d, ky) / dz;
+    w.Sz = 1.0f / dz;
+    return w;
+}
+
+__device__ __forceinline__ bool hit_triangle(const WRay& w, v3 v0, v3 v1, v3 v2, float tmax, float* t_out,
+                                             float* b1, float* b2) {
+    const v3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
+    float Az = pick(A, w.kz), Bz = pick(B, w.kz), Cz = pick(C, w.kz);
+    const float Ax = pick(A, w.kx) - w.Sx * Az, Ay = pick(A, w.ky) - w.Sy * Az;
+    const float Bx = pick(B, w.kx) - w.Sx * Bz, By = pick(B, w.ky) - w.Sy * Bz;
+    const float Cx = pick(C, w.kx) - w.Sx * Cz, Cy = pick(C, w.ky) - w.Sy * Cz;
+    float U = Cx * By - Cy * Bx;
+    float Vv = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || Vv == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        Vv = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
+    const float det = U + Vv + W;
+    if (det == 0.0f) return false;
+    Az = w.Sz * Az;
+    Bz = w.Sz * Bz;
+    Cz = w.Sz * Cz;
+    const float T = U * Az + Vv * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= 0.0f) || !(t <= tmax)) return false;
+    *t_out = t;
+    *b1 = Vv * rcp;
+    *b2 = W * rcp;
+    return true;
+}
+
+struct SlabRay {
+    float ix, iy, iz, ox, oy, oz;
+};
+
+__device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
+    SlabRay r;
+    r.ix = safe_rcp(d.x);
+    r.iy = safe_rcp(d.y);
+    r.iz = safe_rcp(d.z);
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    return r;
+}
+
+// Box-test the four children of one node against [0, bound]; push the hit
+// ones farthest first and return the nearest (or the next stack entry).
+__device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ nodes, uint32_t index, const SlabRay& r,
+                                               float bound, Stack& st) {
+    const BvhNode4 n = nodes[index];
+    const float lx[4] = {n.lo_x.x, n.lo_x.y, n.lo_x.z, n.lo_x.w};
+    const float hx[4] = {n.hi_x.x, n.hi_x.y, n.hi_x.z, n.hi_x.w};
+    const float ly[4] = {n.lo_y.x, n.lo_y.y, n.lo_y.z, n.lo_y.w};
+    const float hy[4] = {n.hi_y.x, n.hi_y.y, n.hi_y.z, n.hi_y.w};
+    const float lz[4] = {n.lo_z.x, n.lo_z.y, n.lo_z.z, n.lo_z.w};
+    const float hz[4] = {n.hi_z.x, n.hi_z.y, n.hi_z.z, n.hi_z.w};
+    const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+    uint32_t hitc[4];
+    float hitd[4];
+    int nh = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float ax = fmaf(lx[c], r.ix, -r.ox), bx = fmaf(hx[c], r.ix, -r.ox);
+        const float ay = fmaf(ly[c], r.iy, -r.oy), by = fmaf(hy[c], r.iy, -r.oy);
+        const float az = fmaf(lz[c], r.iz, -r.oz), bz = fmaf(hz[c], r.iz, -r.oz);
+        const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+        const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound)) * 1.0000004f;
+        if (t0 <= t1 && ch[c] != CHILD_EMPTY) {
+            int k = nh++;
+            while (k > 0 && hitd[k - 1] > t0) {
+                hitd[k] = hitd[k - 1];
+                hitc[k] = hitc[k - 1];
+                --k;
+            }
+            hitd[k] = t0;
+            hitc[k] = ch[c];
+        }
+    }
+    if (nh == 0) return st.pop_or_done();
+    for (int k = nh - 1; k > 0; --k) st.push(hitc[k]);
+    return hitc[0];
+}
+
+// The speculative while-while walk over [0, bound]. `leaf(ref)` tests one
+// leaf's triangles (it may shrink `bound`) and returns true to end the ray.
+template <typename Leaf>
+__device__ __forceinline__ void walk(const BvhNode4* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
+                                     uint32_t& nodes_visited, Leaf&& leaf_fn) {
+    st.sp = 0;
+    uint32_t cur = 0;  // the root is always an internal node
+    uint32_t leaf = NODE_DONE;
+    while (true) {
+        while (!(cur & CHILD_LEAF)) {
+            cur = visit_node(nodes, cur, r, bound, st);
+            ++nodes_visited;
+            if ((cur & CHILD_LEAF) && cur != NODE_DONE && leaf == NODE_DONE) {
+                leaf = cur;
+                cur = st.pop_or_done();
+            }
+            if (!__any(leaf == NODE_DONE)) break;  // every lane still here holds a leaf
+        }
+        while (leaf != NODE_DONE) {
+            if (leaf_fn(leaf)) return;
+            leaf = NODE_DONE;
+            if ((cur & CHILD_LEAF) && cur != NODE_DONE) {
+                leaf = cur;
+                cur = st.pop_or_done();
+            }
+        }
+        if (cur == NODE_DONE) return;
+    }
+}
+
+// A resumable walk for persistent lanes: the node to descend next and the
+// postponed leaf. The ray is done when both are NODE_DONE.
+struct WalkState {
+    uint32_t cur, leaf;
+    __device__ __forceinline__ bool done() const { return cur == NODE_DONE && leaf == NODE_DONE; }
+};
+
+// One while-while round of `walk`: descend until every lane still
+// descending holds a postponed leaf, then test leaves until the next one is
+// an internal node.
+template <typename Leaf>
+__device__ __forceinline__ void walk_round(const BvhNode4* __restrict__ nodes, const SlabRay& r, float& bound,
+                                           Stack& st, WalkState& ws, uint32_t& nodes_visited, Leaf&& leaf_fn) {
+    uint32_t cur = ws.cur, leaf = ws.leaf;
+    while (!(cur & CHILD_LEAF)) {
+        cur = visit_node(nodes, cur, r, bound, st);
+        ++nodes_visited;
+        if ((cur & CHILD_LEAF) && cur != NODE_DONE && leaf == NODE_DONE) {
+            leaf = cur;
+            cur = st.pop_or_done();
+        }
+        if (!__any(leaf == NODE_DONE)) break;
+    }
+    while (leaf != NODE_DONE) {
+        if (leaf_fn(leaf)) {
+            cur = NODE_DONE;
+            leaf = NODE_DONE;
+            break;
+        }
+        leaf = NODE_DONE;
+        if ((cur & CHILD_LEAF) && cur != NODE_DONE) {
+            leaf = cur;
+            cur = st.pop_or_done();
+        }
+    }
+    ws.cur = cur;
+    ws.leaf = leaf;
+}
+
+// Wave-private work distribution for persistent lanes: the wave claims
+// chunks of queue indices with one atomic each and hands them to its idle
+// lanes, so a long ray no longer holds 63 finished lanes hostage.
+struct WaveFetch {
+    uint32_t* counter;
+    uint32_t total;
+    uint32_t next = 0, end = 0;
+    bool exhausted = false;
+    static constexpr uint32_t CHUNK = 64;
+    // For every lane with `idle` set: an index into the queue, or
+    // 0xffffffff when the queue is drained (the lane is then finished for
+    // good); lanes that get neither wait for the next call.
+    __device__ __forceinline__ uint32_t take(bool idle, bool* drained) {
+        *drained = false;
+        const uint64_t need = __ballot(idle);
+        if (need == 0) return 0xffffffffu;
+        const uint32_t lane = __lane_id();
+        if (next >= end && !exhausted) {
+            const uint32_t leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t c = 0;
+            if (lane == leader) c = atomicAdd(counter, CHUNK);
+            c = __shfl(c, leader);
+            next = c;
+            end = c + CHUNK < total ? c + CHUNK : total;
+            if (c + CHUNK >= total) exhausted = true;
+        }
+        const uint32_t avail = end > next ? end - next : 0;
+        const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
+        const uint32_t granted = __popcll(need) < avail ? __popcll(need) : avail;
+        uint32_t index = 0xffffffffu;
+        if (idle) {
+            if (rank < avail) index = next + rank;
+            else if (exhausted) *drained = true;
+        }
+        next += granted;
+        return index;
+    }
+};
+
+__device__ __forceinline__ void leaf_range(uint32_t ref, uint32_t* first, uint32_t* count) {
+    *first = ref & LEAF_FIRST_MASK;
+    *count = ((ref >> 27) & 15u) + 1u;
+}
+
+struct TraceHit {
+    bool hit;
+    float t;
+    uint32_t inst, prim;
+    float b1, b2;
+};
+
+// Closest hit among triangles whose instance mask meets `mask`. Crossings
+// of fractional-opacity instances commit stochastically
+// (intersect.slang:172-188): hashUniform(pixel, sample, bounce, crossing) <
+// opacity, or once TRANSPARENT_CAP crossings have passed. Equal-t ties go
+// to the lower (instance, primitive).
+// The closest-hit leaf test (a crossing of a fractional-opacity instance
+// draws its commit from the pixel's hash stream).
+__device__ __forceinline__ void closest_leaf(const SceneView& s, const WRay& w, uint32_t ref, uint32_t mask,
+                                             uint32_t pixel, uint32_t sample, uint32_t bounce, TraceHit& best,
+                                             uint32_t& crossings, uint32_t& tris_tested) {
+    uint32_t first, count;
+    leaf_range(ref, &first, &count);
+    for (uint32_t k = 0; k < count; ++k) {
+        const BvhTriangle tri = s.tris[first + k];
+        ++tris_tested;
+        const uint32_t flags = __float_as_uint(tri.v2.w);
+        if (((flags >> 8) & mask) == 0) continue;
+        float t, b1, b2;
+        if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), best.t, &t, &b1, &b2)) continue;
+        const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
+        if (best.hit && t == best.t && (inst > best.inst || (inst == best.inst && prim > best.prim))) continue;
+        if (flags & TRI_NON_OPAQUE) {
+            const float opacity = s.materials[inst].opacity;
+            const bool commit = crossings >= TRANSPARENT_CAP || hash_uniform(pixel, sample, bounce, crossings) < opacity;
+            ++crossings;
+            if (!commit) continue;
+        }
+        best = TraceHit{true, t, inst, prim, b1, b2};
+    }
+}
+
+__device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d, float tmax, uint32_t mask,
+                                                  Stack& st, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                                                  uint32_t& nodes_visited, uint32_t& tris_tested) {
+    const SlabRay r = make_slab(o, d);
+    const WRay w = make_wray(o, d);
+    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
+    uint32_t crossings = 0;
+    walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
+        closest_leaf(s, w, ref, mask, pixel, sample, bounce, best, crossings, tris_tested);
+        return false;
+    });
+    return best;
+}
+
+// The shadow leaf test: returns true when a solid non-target hit nearer
+// than t_early ends the ray.
+__device__ __forceinline__ bool visibility_leaf(const SceneView& s, const WRay& w, uint32_t ref, uint32_t target_inst,
+                                                uint32_t target_prim, float t_early, TraceHit& best,
+                                                bool& translucent, uint32_t& tris_tested) {
+    uint32_t first, count;
+    leaf_range(ref, &first, &count);
+    for (uint32_t k = 0; k < count; ++k) {
+        const BvhTriangle tri = s.tris[first + k];
+        ++tris_tested;
+        float t, b1, b2;
+        if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), best.t, &t, &b1, &b2)) continue;
+        const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
+        if (best.hit && t == best.t && (inst > best.inst || (inst == best.inst && prim > best.prim))) continue;
+        if (!(inst == target_inst && prim == target_prim)) {
+            if (__float_as_uint(tri.v2.w) & TRI_NON_OPAQUE) {
+                translucent = true;
+                continue;
+            }
+            if (t < t_early) return true;
+        }
+        best = TraceHit{true, t, inst, prim, b1, b2};
+    }
+    return false;
+}
+
+__device__ inline float shadow_attenuation(const SceneView& s, const SlabRay& r, const WRay& w, float t_end,
+                                           uint32_t target_inst, uint32_t target_prim, Stack& st,
+                                           uint32_t& nodes_visited);
+
+// A next-event connection's visibility (trace_shadow.slang:41-71): visible
+// iff nothing solid commits within tmax or the committed hit is the target
+// (instance, primitive). Returns the attenuation (0 when occluded).
+// Fractional-opacity crossings in front of the connection's end multiply
+// their transparency in, nearest first — the reference multiplies them in
+// the driver's traversal order, which no test pins; nearest-first is an
+// order any traversal reproduces. Only the first ATTENUATION_CROSSINGS
+// count. A solid non-target hit nearer than `t_early` occludes without
+// looking further.
+__device__ __forceinline__ float trace_visibility(const SceneView& s, v3 o, v3 d, float tmax, uint32_t target_inst,
+                                                  uint32_t target_prim, float t_early, Stack& st,
+                                                  uint32_t& nodes_visited, uint32_t& tris_tested) {
+    const SlabRay r = make_slab(o, d);
+    const WRay w = make_wray(o, d);
+    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
+    bool occluded = false, translucent = false;
+    walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
+        occluded = visibility_leaf(s, w, ref, target_inst, target_prim, t_early, best, translucent, tris_tested);
+        return occluded;
+    });
+    if (occluded) return 0.0f;
+    if (best.hit && !(best.inst == target_inst && best.prim == target_prim)) return 0.0f;
+    if (!translucent) return 1.0f;
+    return shadow_attenuation(s, r, w, best.hit ? best.t : tmax, target_inst, target_prim, st, nodes_visited);
+}
+
+// The rare path of a visible connection that crossed fractional-opacity
+// geometry: one walk per crossing, each finding the nearest crossing past
+// the previous one, nearest first, at most ATTENUATION_CROSSINGS of them.
+__device__ inline float shadow_attenuation(const SceneView& s, const SlabRay& r, const WRay& w, float t_end,
+                                           uint32_t target_inst, uint32_t target_prim, Stack& st,
+                                           uint32_t& nodes_visited) {
+    float attenuation = 1.0f;
+    float pt = -1.0f;
+    uint32_t pi = 0, pp = 0;
+    for (uint32_t round = 0; round < ATTENUATION_CROSSINGS; ++round) {
+        float bound = t_end;
+        bool found = false;
+        uint32_t fi = 0, fp = 0;
+        walk(s.nodes, r, bound, st, nodes_visited, [&](uint32_t ref) {
+            uint32_t first, count;
+            leaf_range(ref, &first, &count);
+            for (uint32_t k = 0; k < count; ++k) {
+                const BvhTriangle tri = s.tris[first + k];
+                if (!(__float_as_uint(tri.v2.w) & TRI_NON_OPAQUE)) continue;
+                const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
+                if (inst == target_inst && prim == target_prim) continue;
+                float t, b1, b2;
+                if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), bound, &t, &b1, &b2)) continue;
+                if (!(t < t_end)) continue;
+                const bool after_prev = t > pt || (t == pt && (inst > pi || (inst == pi && prim > pp)));
+                if (!after_prev) continue;
+                const bool before_best = !found || t < bound || (t == bound && (inst < fi || (inst == fi && prim < fp)));
+                if (!before_best) continue;
+                found = true;
+                bound = t;
+                fi = inst;
+                fp = prim;
+            }
+            return false;
+        });
+        if (!found) break;
+        attenuation *= 1.0f - s.materials[fi].opacity;
+        pt = bound;
+        pi = fi;
+        pp = fp;
+    }
+    return attenuation;
+}
+
+}  // namespace cn

@@ -1635,6 +1635,9 @@ inline void add4(f4& a, f3 v, float w) {
 
 constexpr uint32_t ROULETTE_START = 3;  // shade_surface.slang:53
 constexpr uint32_t TRANSPARENT_CAP = 8;  // intersect.slang:23
+// The shadow connection multiplies in the transparency of at most this many
+// fractional-opacity crossings, nearest first.
+constexpr size_t ATTENUATION_CROSSINGS = 16;
 
 // One path of one wave: raygen.slang:64-112, then per bounce
 // intersect.slang:45-88, shade_miss.slang:27-75 or
@@ -1840,29 +1843,31 @@ void trace_path(const orc_scene& s, const Basis& basis, float aperture, uint32_t
         auto traceShadow = [&]() {
             if (!haveShadow) return;
             struct Crossing {
-                float t, transparency;
+                float t;
+                uint32_t inst, prim;
             };
-            Crossing list[32];
-            int n = 0;
-            float overflow = 1.0f;
+            std::vector<Crossing> list;
             Ray sray{shadowOrigin, shadowDir, shadowTmax};
             HitRecord sh = closest_hit(s, sray, MASK_ALL, [&](uint32_t i, uint32_t p, float t) {
                 if (i == shadowLight && p == shadowPrim) return true;
-                float transparency = 1.0f - s.materials[i].opacity;
-                if (n < 32) list[n++] = {t, transparency};
-                else overflow *= transparency;
+                list.push_back({t, i, p});
                 return false;
             });
             if (sh.hit && !(sh.instance == shadowLight && sh.primitive == shadowPrim)) return;
             float tEnd = sh.hit ? sh.t : shadowTmax;
-            std::sort(list, list + n, [](const Crossing& a, const Crossing& b) {
-                return a.t < b.t || (a.t == b.t && a.transparency < b.transparency);
+            std::vector<Crossing> front;
+            for (const Crossing& c : list) {
+                if (c.t < tEnd) front.push_back(c);
+            }
+            std::sort(front.begin(), front.end(), [](const Crossing& a, const Crossing& b) {
+                if (a.t != b.t) return a.t < b.t;
+                if (a.inst != b.inst) return a.inst < b.inst;
+                return a.prim < b.prim;
             });
             float attenuation = 1.0f;
-            for (int k = 0; k < n; ++k) {
-                if (list[k].t < tEnd) attenuation *= list[k].transparency;
+            for (size_t k = 0; k < front.size() && k < ATTENUATION_CROSSINGS; ++k) {
+                attenuation *= 1.0f - s.materials[front[k].inst].opacity;
             }
-            attenuation *= overflow;
             if (attenuation < 1e-4f) return;
             add4(out.radiance, shadowContribution * attenuation, 0.0f);
         };
