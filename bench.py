@@ -71,8 +71,8 @@ def cpu_baseline(data, width, height, max_bounces, budget_s=12.0):
 
     threads = max(1, min(16, os.cpu_count() or 1))
     scene = orc.OracleScene(data)
-    # Probe a band of rows, then trace whole frames at successive sample
-    # indices (or a band, if one frame is over budget) until ~budget_s.
+    # Probe a band of rows; then trace whole frames at successive sample
+    # indices until budget_s has passed (or one band, if a frame would not fit).
     t0 = time.perf_counter()
     scene.trace_rows(width, height, 0, 16, 0, max_bounces=max_bounces, threads=threads)
     frame_s = (time.perf_counter() - t0) * height / 16
@@ -85,10 +85,11 @@ def cpu_baseline(data, width, height, max_bounces, budget_s=12.0):
         paths = rows * width
         sample = f"rows {row0}..{row0 + rows} of one {width}x{height} wave (sample 0)"
     else:
-        waves = max(1, int(budget_s / max(frame_s, 1e-3)))
+        waves = 0
         t0 = time.perf_counter()
-        for s in range(waves):
-            scene.trace(width, height, s, max_bounces=max_bounces, threads=threads)
+        while waves == 0 or time.perf_counter() - t0 < budget_s:
+            scene.trace(width, height, waves, max_bounces=max_bounces, threads=threads)
+            waves += 1
         elapsed = time.perf_counter() - t0
         paths = waves * width * height
         sample = f"{waves} full {width}x{height} waves (samples 0..{waves - 1})"
@@ -210,8 +211,8 @@ def main():
                         + (", AOVs" if aov else ""),
             "triangles": data.triangle_count,
             "pixels": pixels,
-            "path_pool": cn.DEFAULT_CAPACITY,
-            "form": "plain (one progressive frame per step)",
+            "path_pool": cn.POOL_CAPACITY,
+            "form": "plain (one progressive frame, one sample per step; waves of several samples)",
         },
         "grays_per_s": round(grays, 4),
         "kernel_ms_per_step": kernel_ms,

@@ -6,6 +6,6 @@ in-tree by ``cenote_amd.build``); this package is the Python host harness
 over it.
 """
 
-from ._abi import CenoteError, DEFAULT_CAPACITY, DEFAULT_MAX_BOUNCES  # noqa: F401
+from ._abi import CenoteError, DEFAULT_CAPACITY, DEFAULT_MAX_BOUNCES, POOL_CAPACITY  # noqa: F401
 from .render import Context, Film, LightSampling, Renderer, Scene  # noqa: F401
 from .scene import CameraData, Material, MeshData, OpenPBR, SceneData  # noqa: F401

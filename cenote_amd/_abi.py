@@ -31,6 +31,7 @@ LIGHT_POINT = 2
 TEXTURE_NONE = 0xFFFFFFFF
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
+POOL_CAPACITY = 1 << 24
 
 f3 = C.c_float * 3
 
@@ -175,6 +176,16 @@ class TraversalCounts(C.Structure):
     ]
 
 
+class ViewJob(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("film", C.c_void_p),
+        ("camera", Camera),
+        ("samples", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
 class KernelTimes(C.Structure):
     _fields_ = [
         ("kernels", C.c_uint32),
@@ -219,6 +230,7 @@ SIGNATURES = {
     "cn_film_samples": [_P, _UP],
     "cn_accumulate": [_P, _P, _P],
     "cn_accumulate_async": [_P, _P, _P, C.c_uint32],
+    "cn_render_batch": [_P, C.POINTER(ViewJob), C.c_uint32],
     "cn_renderer_synchronize": [_P],
     "cn_film_averages": [_P, _FP, _FP, _FP, _FP],
     "cn_film_sums": [_P, _FP, _FP, _FP, _FP],

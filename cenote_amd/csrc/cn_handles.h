@@ -1,0 +1,152 @@
+// cn_handles.h — the opaque handles behind the C ABI and the host-side
+// helpers the ABI's translation units share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cenote_hip.h"
+#include "cn_launch.h"
+#include "cn_prep.h"
+#include "cn_records.h"
+
+namespace cn {
+
+// Sets the thread-local message cn_last_error() returns; returns `status`.
+int fail(int status, const char* fmt, ...);
+
+#define CN_HIP(call)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess) {                                                                            \
+            return ::cn::fail(e_ == hipErrorOutOfMemory ? CN_E_ALLOC : CN_E_DEVICE_FAULT, "%s: %s (%s:%d)", \
+                              #call, hipGetErrorString(e_), __FILE__, __LINE__);                          \
+        }                                                                                                  \
+    } while (0)
+
+// A device allocation that frees itself.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        count = n;
+        if (n == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
+    }
+    hipError_t upload(const T* src, size_t n) {
+        hipError_t e = alloc(n);
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpy(ptr, src, n * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+// Per-kernel timing: a HIP event pair on the launch stream around every
+// launch, read back and summed on demand.
+struct EventProfiler final : Profiler {
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> open;
+    std::vector<std::tuple<int, size_t, size_t>> pairs;
+    size_t used = 0;
+    double total_ms[K_KINDS] = {};
+    uint64_t launches[K_KINDS] = {};
+
+    hipEvent_t next();
+    void begin(int kind) override;
+    void end(int kind) override;
+    void collect();
+    ~EventProfiler() override;
+};
+
+}  // namespace cn
+
+struct cn_context {
+    int device = 0;
+    uint32_t compute_units = 0;
+    std::string name;
+    cn::DevBuf<float> tables;  // the BSDF tables, shared by every scene
+};
+
+struct cn_scene {
+    cn_context* ctx = nullptr;
+    cn_camera camera{};
+    cn::DevBuf<cn::BvhNode4> nodes;
+    cn::DevBuf<cn::BvhTriangle> tris;
+    cn::DevBuf<cn::InstanceRecord> instances;
+    cn::DevBuf<cn_material> materials;
+    cn::DevBuf<cn_light_record> lights;
+    cn::DevBuf<float> positions, normals;
+    cn::DevBuf<uint32_t> indices;
+    cn::DevBuf<float4> env_texels;
+    cn::DevBuf<float> env_marginal, env_conditional, env_pdfs;
+    std::vector<cn_light_record> host_lights;
+    cn::EnvironmentTables host_env;
+    cn_scene_info info{};
+    cn::SceneView view{};
+};
+
+struct cn_film {
+    cn_context* ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    uint32_t samples = 0;
+    cn::DevBuf<float4> beauty_sum, albedo_sum, normal_sum;
+    cn::DevBuf<float> depth_sum;
+    // A one-sample staging set for cn_film_set_sample / cn_film_accumulate_sample.
+    cn::DevBuf<float4> beauty, albedo, normal;
+    cn::DevBuf<float> depth;
+    // Resolve outputs, allocated on first use.
+    cn::DevBuf<float4> out_beauty, out_albedo, out_normal;
+    cn::DevBuf<float> out_depth;
+};
+
+// Job tables in flight: a ring of device tables, each filled from pinned
+// host staging and released by an event once its wave has finished.
+constexpr uint32_t JOB_RING = 32;
+
+struct cn_renderer {
+    cn_context* ctx = nullptr;
+    std::unique_ptr<cn::EventProfiler> profiler;
+    uint32_t capacity = 0;
+    uint32_t max_bounces = 0;
+    uint32_t light_sampling = 0;
+    hipStream_t stream = nullptr;
+    cn::LaunchShape shape{};
+    int counting = 0;
+    // path pool
+    cn::DevBuf<float4> origin, direction, throughput;
+    cn::DevBuf<uint4> hit;
+    cn::DevBuf<uint2> key;
+    cn::DevBuf<uint32_t> state, ray_queue0, ray_queue1, hit_queue, miss_queue;
+    cn::DevBuf<float4> shadow_origin, shadow_dir, shadow_contrib;
+    cn::DevBuf<uint32_t> shadow_prim, spill;
+    cn::DevBuf<unsigned long long> traversal;
+    cn::DevBuf<uint32_t> counters;
+    // slot-indexed wave targets
+    cn::DevBuf<float4> radiance, albedo, normal, guide;
+    cn::DevBuf<float> depth;
+    // job-table ring
+    cn::DevBuf<cn::JobRecord> jobs;
+    cn::JobRecord* staging = nullptr;  // pinned, JOB_RING * MAX_JOBS
+    hipEvent_t ring_event[JOB_RING] = {};
+    bool ring_used[JOB_RING] = {};
+    uint32_t ring_next = 0;
+    uint32_t last_ranges = 0;
+    cn::PathPool pool{};
+};

@@ -47,16 +47,21 @@ struct Profiler {
     virtual ~Profiler() = default;
 };
 
-hipError_t launch_raygen(hipStream_t stream, const CameraView& cam, const PathPool& p, const WaveTargets& tg,
-                         uint32_t* counters, uint32_t counter_count, uint32_t width, uint32_t height, uint32_t base,
-                         uint32_t count, uint32_t sample, Profiler* prof);
+// Camera paths for slots [base, base + count) of a wave whose jobs (sorted
+// by slot base) are `jobs[0 .. job_count)`.
+hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_count, uint32_t base,
+                         uint32_t count, Profiler* prof);
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, uint32_t sample,
-                         int counting, Profiler* prof);
+                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, int counting,
+                         Profiler* prof);
 
-hipError_t launch_accumulate(hipStream_t stream, const FilmView& f, uint32_t pixels, uint32_t reset, Profiler* prof);
+// Fold every job's slots into its film sums (jobs sorted by pixel base;
+// total_pixels is the sum of the jobs' view pixels).
+hipError_t launch_accumulate(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const WaveTargets& tg,
+                             uint32_t total_pixels, Profiler* prof);
 
 // Divides the sums into the FilmView's sample-buffer slots (the caller
 // passes the resolve targets there).

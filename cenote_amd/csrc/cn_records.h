@@ -110,14 +110,17 @@ struct SceneView {
     float env_select_prob;
 };
 
-// The SoA path pool (shaders/pathstate.slang:207-232). A path's pixel is
-// its range base plus its slot, so no pixel field is stored.
+// The SoA path pool (shaders/pathstate.slang:207-232). A wave is a set of
+// jobs — (view, consecutive samples) pairs — laid end to end in a slot
+// space; a path's target slot is its range base plus its pool index, and
+// its sampler key (pixel within its view, sample index) rides in `key`.
 struct PathPool {
     float4* origin;       // xyz origin
     float4* direction;    // xyz direction, w = cosine to the camera forward
     uint4* hit;           // instance, primitive, b1 bits, b2 bits
     float4* throughput;   // xyz weight, w = previous scatter pdf
     uint32_t* state;      // lobe << 24 | medium
+    uint2* key;           // pixel within the view, sample index
     uint32_t* ray_queue[2];  // ping-pong ray queues for bounces >= 1
     uint32_t* hit_queue;
     uint32_t* miss_queue;
@@ -130,8 +133,8 @@ struct PathPool {
     unsigned long long* traversal_counts;  // 4 counters when enabled
 };
 
-// The wave's per-pixel targets (shaders/pathstate.slang:241-258 plus the
-// radiance accumulator).
+// The wave's per-slot targets (shaders/pathstate.slang:241-258 plus the
+// radiance accumulator): one entry per (view pixel, sample) of the wave.
 struct WaveTargets {
     float4* radiance;
     float4* albedo;  // NULL when AOVs are off
@@ -148,5 +151,27 @@ struct CameraView {
     float3 forward;
     float aperture;
 };
+
+// One job of a wave: `samples` consecutive samples, starting at
+// `first_sample`, of every pixel of one view, occupying slots
+// [slot_base, slot_base + samples * pixels) sample-major. When `beauty_sum`
+// is set, the accumulate kernel folds the job's slots into that film's sums
+// in sample order (render/film.rs:66-127, accumulate.slang:46-67).
+constexpr uint32_t JOB_RESET = 1u;  // the job's first sample overwrites the sums
+
+struct JobRecord {
+    CameraView cam;
+    uint32_t width, height, pixels;
+    uint32_t slot_base, pixel_base;
+    uint32_t first_sample, samples;
+    uint32_t flags;
+    float4* beauty_sum;
+    float4* albedo_sum;
+    float4* normal_sum;
+    float* depth_sum;
+};
+
+// Jobs one wave may carry.
+constexpr uint32_t MAX_JOBS = 256;
 
 }  // namespace cn

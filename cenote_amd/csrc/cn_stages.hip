@@ -103,35 +103,48 @@ __device__ __forceinline__ void camera_ray(const CameraView& cam, uint32_t width
     *cos_forward = dot(d, normalize(forward));
 }
 
-// One thread per pixel of the range. Also zero-fills the wave's per-pixel
-// accumulators (the reference's Pass::Fill, wavefront.rs:733-759) and
-// resets the range's queue counters: raygen owns every pixel exactly once.
-__global__ __launch_bounds__(BLOCK) void k_raygen(CameraView cam, PathPool p, WaveTargets tg, uint32_t* counters,
-                                                  uint32_t counter_count, uint32_t width, uint32_t height,
-                                                  uint32_t base, uint32_t count, uint32_t sample) {
+// One thread per slot of the range: find the slot's job (jobs sorted by
+// slot base), derive its pixel and sample, and start the camera path. Also
+// zero-fills the slot's accumulators (the reference's Pass::Fill,
+// wavefront.rs:733-759) and resets the range's queue counters: raygen owns
+// every slot exactly once.
+__global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ jobs, uint32_t job_count, PathPool p,
+                                                  WaveTargets tg, uint32_t* counters, uint32_t counter_count,
+                                                  uint32_t base, uint32_t count) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < counter_count) counters[i] = 0;
     if (i >= count) return;
-    const uint32_t pixel = base + i;
+    const uint32_t slot = base + i;
+    uint32_t lo = 0, hi = job_count;
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].slot_base <= slot) lo = mid;
+        else hi = mid;
+    }
+    const JobRecord& job = jobs[lo];
+    const uint32_t local = slot - job.slot_base;
+    const uint32_t k = local / job.pixels;
+    const uint32_t pixel = local - k * job.pixels;
+    const uint32_t sample = job.first_sample + k;
     v3 o, d;
     float cf;
-    camera_ray(cam, width, height, pixel, sample, &o, &d, &cf);
+    camera_ray(job.cam, job.width, job.height, pixel, sample, &o, &d, &cf);
     p.origin[i] = F4(o, 0.0f);
     p.direction[i] = F4(d, cf);
     p.throughput[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    tg.radiance[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.key[i] = make_uint2(pixel, sample);
+    tg.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (tg.albedo) {
-        tg.albedo[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        tg.normal[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        tg.depth[pixel] = 0.0f;
+        tg.albedo[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tg.normal[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tg.depth[slot] = 0.0f;
     }
 }
 
 // --- intersect (intersect.slang:45-88) ----------------------------------------------
 
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_intersect(SceneView s, PathPool p, uint32_t* counters, uint32_t base,
-                                                     uint32_t range_count, uint32_t bounce, uint32_t sample,
-                                                     int counting) {
+                                                     uint32_t range_count, uint32_t bounce, int counting) {
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
     const uint32_t n = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
     const uint32_t* rays = p.ray_queue[bounce & 1];
@@ -148,7 +161,8 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_intersect(SceneView s, Path
             path = bounce == 0 ? i : rays[i];
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
-            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, base + path, sample, bounce, nodes, tris);
+            const uint2 key = p.key[path];
+            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, nodes, tris);
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
         }
@@ -270,7 +284,7 @@ constexpr uint32_t SHADE_BLOCK = 1024;
 
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                          uint32_t base, uint32_t bounce, uint32_t max_bounces,
-                                                         uint32_t light_sampling, uint32_t sample) {
+                                                         uint32_t light_sampling) {
     __shared__ float lds_tables[T_FLOATS];
     const uint32_t n = counters[bounce * Q_COUNT + Q_HIT];
     if (blockIdx.x * SHADE_BLOCK >= n) return;  // uniform per block: nothing to shade here
@@ -292,7 +306,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
         uint32_t sh_light = LIGHT_NONE, sh_prim = 0;
         if (valid) {
             path = p.hit_queue[i];
-            pixel = base + path;
+            pixel = base + path;  // the path's target slot
+            const uint2 key = p.key[path];
             const uint4 hit = p.hit[path];
             const float4 dir4 = p.direction[path];
             const v3 direction = xyz(dir4);
@@ -352,8 +367,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
                 // Next-event estimation (shade_surface.slang:363-446).
                 const float envProb = s.env_select_prob;
                 if (light_sampling != CN_LIGHT_SAMPLING_BSDF && (s.light_count > 0 || envProb > 0.0f)) {
-                    const float uLight = sample_1d(pixel, sample, bounce_dim(bounce, SLOT_NEE_LIGHT));
-                    const f2 uPoint = sample_2d(pixel, sample, bounce_dim(bounce, SLOT_NEE_POINT));
+                    const float uLight = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_LIGHT));
+                    const f2 uPoint = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_POINT));
                     v3 unshadowed = S3(0.0f);
                     if (uLight < envProb) {
                         const EnvSample cand = env_sample(s, uPoint);
@@ -398,8 +413,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
                 }
 
                 // Scatter (shade_surface.slang:453-489).
-                const float uLobe = sample_1d(pixel, sample, bounce_dim(bounce, SLOT_BSDF_LOBE));
-                const f2 u = sample_2d(pixel, sample, bounce_dim(bounce, SLOT_BSDF_DIR));
+                const float uLobe = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_LOBE));
+                const f2 u = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_DIR));
                 const BsdfSample scatter = sample_bsdf(closure, wo, uLobe, u);
                 const v3 wiWorld = to_world(frame, scatter.wi);
                 v3 scattered = S3(0.0f);
@@ -413,7 +428,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
                 } else {
                     if (bounce >= ROULETTE_START) {
                         survival = fminf(1.0f, survival);
-                        if (sample_1d(pixel, sample, bounce_dim(bounce, SLOT_RR)) >= survival) terminal = true;
+                        if (sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_RR)) >= survival) terminal = true;
                         else scattered = scattered / survival;
                     }
                 }
@@ -500,17 +515,51 @@ __device__ __forceinline__ float4 guarded(float4 c) {
 
 __device__ __forceinline__ float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 
-__global__ __launch_bounds__(BLOCK) void k_accumulate(FilmView f, uint32_t pixels, uint32_t reset) {
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pixels; i += gridDim.x * BLOCK) {
-        const float4 b = guarded(f.beauty[i]);
-        f.beauty_sum[i] = reset ? b : add4(f.beauty_sum[i], b);
-        const float4 a = guarded(f.albedo[i]);
-        f.albedo_sum[i] = reset ? a : add4(f.albedo_sum[i], a);
-        const float4 nn = guarded(f.normal[i]);
-        f.normal_sum[i] = reset ? nn : add4(f.normal_sum[i], nn);
-        float d = f.depth[i];
-        if (isnan(d)) d = 0.0f;
-        f.depth_sum[i] = reset ? d : f.depth_sum[i] + d;
+// One thread per view pixel of the wave: fold the job's samples into its
+// film's sums in sample order, reading the sums once.
+__global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restrict__ jobs, uint32_t job_count,
+                                                      WaveTargets tg, uint32_t total_pixels) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total_pixels; i += gridDim.x * BLOCK) {
+        uint32_t lo = 0, hi = job_count;
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (jobs[mid].pixel_base <= i) lo = mid;
+            else hi = mid;
+        }
+        const JobRecord& job = jobs[lo];
+        if (!job.beauty_sum) continue;
+        const uint32_t px = i - job.pixel_base;
+        const bool reset = job.flags & JOB_RESET;
+        float4 b = reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : job.beauty_sum[px];
+        const bool aov = tg.albedo != nullptr && job.albedo_sum != nullptr;
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nn = a;
+        float d = 0.0f;
+        if (aov && !reset) {
+            a = job.albedo_sum[px];
+            nn = job.normal_sum[px];
+            d = job.depth_sum[px];
+        }
+        for (uint32_t k = 0; k < job.samples; ++k) {
+            const uint32_t slot = job.slot_base + k * job.pixels + px;
+            const bool first = reset && k == 0;
+            const float4 cb = guarded(tg.radiance[slot]);
+            b = first ? cb : add4(b, cb);
+            if (aov) {
+                const float4 ca = guarded(tg.albedo[slot]);
+                a = first ? ca : add4(a, ca);
+                const float4 cn = guarded(tg.normal[slot]);
+                nn = first ? cn : add4(nn, cn);
+                float cd = tg.depth[slot];
+                if (isnan(cd)) cd = 0.0f;
+                d = first ? cd : d + cd;
+            }
+        }
+        job.beauty_sum[px] = b;
+        if (aov) {
+            job.albedo_sum[px] = a;
+            job.normal_sum[px] = nn;
+            job.depth_sum[px] = d;
+        }
     }
 }
 
@@ -567,21 +616,21 @@ static uint32_t grid_for(uint32_t items, uint32_t cap) {
     return blocks < cap ? blocks : cap;
 }
 
-hipError_t launch_raygen(hipStream_t stream, const CameraView& cam, const PathPool& p, const WaveTargets& tg,
-                         uint32_t* counters, uint32_t counter_count, uint32_t width, uint32_t height, uint32_t base,
-                         uint32_t count, uint32_t sample, Profiler* prof) {
+hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_count, uint32_t base,
+                         uint32_t count, Profiler* prof) {
     const uint32_t items = count > counter_count ? count : counter_count;
     if (prof) prof->begin(K_RAYGEN);
-    hipLaunchKernelGGL(k_raygen, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, cam, p, tg, counters,
-                       counter_count, width, height, base, count, sample);
+    hipLaunchKernelGGL(k_raygen, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, jobs, job_count, p, tg,
+                       counters, counter_count, base, count);
     if (prof) prof->end(K_RAYGEN);
     return hipGetLastError();
 }
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, uint32_t sample,
-                         int counting, Profiler* prof) {
+                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, int counting,
+                         Profiler* prof) {
     const uint32_t trace_grid = shape.trace_blocks;
     uint32_t shade_grid = (range_count + SHADE_BLOCK - 1) / SHADE_BLOCK;
     if (shade_grid > shape.shade_blocks) shade_grid = shape.shade_blocks;
@@ -589,7 +638,7 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
     if (prof) prof->begin(K_INTERSECT);
     hipLaunchKernelGGL(k_intersect, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, counters, base, range_count,
-                       bounce, sample, counting);
+                       bounce, counting);
     if (prof) prof->end(K_INTERSECT);
     if (prof) prof->begin(K_SHADE_MISS);
     hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
@@ -597,7 +646,7 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     if (prof) prof->end(K_SHADE_MISS);
     if (prof) prof->begin(K_SHADE_SURFACE);
     hipLaunchKernelGGL(k_shade_surface, dim3(shade_grid), dim3(SHADE_BLOCK), 0, stream, s, p, tg, counters, base,
-                       bounce, max_bounces, light_sampling, sample);
+                       bounce, max_bounces, light_sampling);
     if (prof) prof->end(K_SHADE_SURFACE);
     if (bounce + 1 < max_bounces) {
         if (prof) prof->begin(K_TRACE_SHADOW);
@@ -608,9 +657,11 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate(hipStream_t stream, const FilmView& f, uint32_t pixels, uint32_t reset, Profiler* prof) {
+hipError_t launch_accumulate(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const WaveTargets& tg,
+                             uint32_t total_pixels, Profiler* prof) {
     if (prof) prof->begin(K_ACCUMULATE);
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(pixels, 4096)), dim3(BLOCK), 0, stream, f, pixels, reset);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(total_pixels, 4096)), dim3(BLOCK), 0, stream, jobs, job_count, tg,
+                       total_pixels);
     if (prof) prof->end(K_ACCUMULATE);
     return hipGetLastError();
 }

@@ -161,8 +161,10 @@ class Renderer:
     """The wavefront engine plus the film kernels (Wavefront::new +
     Renderer::with_max_bounces)."""
 
-    def __init__(self, ctx: Context, capacity: int = _abi.DEFAULT_CAPACITY,
+    def __init__(self, ctx: Context, capacity: int = 0,
                  max_bounces: int = _abi.DEFAULT_MAX_BOUNCES, light_sampling: LightSampling = LightSampling.MIS):
+        """``capacity`` paths in the pool; 0 selects the library's pool
+        (``_abi.POOL_CAPACITY``), a multiple of the reference's 2^20."""
         self.ctx = ctx
         self._lib = ctx._lib
         self.max_bounces = max_bounces
@@ -204,6 +206,18 @@ class Renderer:
         check(self._lib.cn_accumulate_async(self.handle, scene.handle, film.handle, count))
         if synchronize:
             self.synchronize()
+
+    def render_batch(self, views):
+        """Many views in shared waves: ``views`` is a sequence of
+        ``(scene, film, camera, samples)``; each film gets ``samples`` more
+        samples of its view, bitwise as sequential ``accumulate`` calls."""
+        jobs = (_abi.ViewJob * max(1, len(views)))()
+        for j, (scene, film, camera, samples) in zip(jobs, views):
+            j.scene = scene.handle.value
+            j.film = film.handle.value
+            j.camera = camera.record()
+            j.samples = samples
+        check(self._lib.cn_render_batch(self.handle, jobs, len(views)))
 
     def accumulate_sample(self, film: Film):
         check(self._lib.cn_film_accumulate_sample(self.handle, film.handle))

@@ -66,6 +66,11 @@ enum {
 /* Wavefront::DEFAULT_CAPACITY and DEFAULT_MAX_BOUNCES (wavefront.rs:462-471). */
 #define CN_DEFAULT_CAPACITY (1u << 20)
 #define CN_DEFAULT_MAX_BOUNCES 8u
+/* The most paths a capacity of 0 lets a wave hold here: 2^24 (228 bytes of
+ * path state and slot targets each, at most 3.8 GB, allocated as views
+ * need it), so that one wave carries 8 samples of a 1080p view and even
+ * the deep-bounce launches fill the chip. */
+#define CN_POOL_CAPACITY (1u << 24)
 
 /*
  * One instance's lowered OpenPBR surface, 144 bytes, field for field the
@@ -261,7 +266,7 @@ int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional,
                         float* pdfs);
 
 /* Wavefront::new / Renderer::with_max_bounces (wavefront.rs:486-540,
- * render/mod.rs:151): capacity paths in the pool (0 = default),
+ * render/mod.rs:151): capacity paths in the pool (0 = CN_POOL_CAPACITY),
  * max_bounces in 1..255, light_sampling a CN_LIGHT_SAMPLING_* value. */
 int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
                        uint32_t light_sampling, cn_renderer** out);
@@ -303,6 +308,22 @@ int cn_accumulate(cn_renderer* renderer, cn_scene* scene, cn_film* film);
  * them; the sums are bitwise those of `count` cn_accumulate calls. */
 int cn_accumulate_async(cn_renderer* renderer, cn_scene* scene, cn_film* film,
                         uint32_t count);
+/* One view of a batch: `samples` more samples of `camera` on `scene`,
+ * folded into `film` (whose size sets the view's resolution). */
+typedef struct cn_view_job {
+    cn_scene* scene;
+    cn_film* film;
+    cn_camera camera;
+    uint32_t samples;
+    uint32_t pad;
+} cn_view_job;
+/* Many views in shared waves (no reference counterpart: the reference
+ * renders one view per Renderer::accumulate, render/mod.rs:245-259).
+ * Each wave packs as many (view, sample) pairs as the path pool holds, in
+ * view order; every film's sums are bitwise those of sequential
+ * cn_accumulate calls on its view. Films must be distinct. Blocks until
+ * done. */
+int cn_render_batch(cn_renderer* renderer, const cn_view_job* views, uint32_t count);
 /* Block until the renderer's stream has drained. */
 int cn_renderer_synchronize(cn_renderer* renderer);
 /* Film::averages (render/film.rs:183-197): sums / samples on the host.

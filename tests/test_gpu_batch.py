@@ -1,0 +1,126 @@
+"""Sample-batched waves and many-view batches: however the engine packs
+(view, sample) pairs into waves, every film's f32 sums must be bit for bit
+those of one sample per wave, one view at a time (render/film.rs:66-127 folds
+sample after sample; accumulate.slang:38-44 adds in that order)."""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import cenote_amd as cn
+from cenote_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    cornell = workloads.cornell()
+    chart = workloads.chart()
+    a = cn.Scene(ctx, cornell)
+    b = cn.Scene(ctx, chart)
+    yield {"cornell": (cornell, a), "chart": (chart, b)}
+    a.close()
+    b.close()
+
+
+def sequential_sums(ctx, scene, camera, width, height, samples, max_bounces):
+    """The plain reference form: one wave per sample, a pool of one view."""
+    r = cn.Renderer(ctx, capacity=width * height, max_bounces=max_bounces)
+    scene.set_camera(camera)
+    film = cn.Film(ctx, width, height)
+    for _ in range(samples):
+        r.accumulate(scene, film)
+    sums = film.sums()
+    film.close()
+    r.close()
+    return sums
+
+
+def assert_same_sums(got, want):
+    for g, w in zip(got, want):
+        equal = np.array_equal(g, w, equal_nan=True)
+        mismatches = int(np.count_nonzero(~((g == w) | (np.isnan(g) & np.isnan(w)))))
+        assert equal, f"{mismatches} of {g.size} values differ"
+
+
+def test_multi_sample_waves_match_one_sample_waves(ctx, scenes):
+    data, scene = scenes["chart"]
+    want = sequential_sums(ctx, scene, data.camera, 160, 90, 9, 12)
+    scene.set_camera(data.camera)
+    # A pool of 4 views' worth: waves of 4, 4 and 1 samples.
+    r = cn.Renderer(ctx, capacity=4 * 160 * 90, max_bounces=12)
+    film = cn.Film(ctx, 160, 90)
+    r.accumulate(scene, film, count=9)
+    assert film.samples == 9
+    got = film.sums()
+    film.close()
+    r.close()
+    assert_same_sums(got, want)
+
+
+def test_waves_split_across_pool_ranges(ctx, scenes):
+    """A pool smaller than one view: every wave walks several ranges."""
+    data, scene = scenes["cornell"]
+    want = sequential_sums(ctx, scene, data.camera, 96, 64, 3, 8)
+    scene.set_camera(data.camera)
+    r = cn.Renderer(ctx, capacity=1000, max_bounces=8)
+    film = cn.Film(ctx, 96, 64)
+    r.accumulate(scene, film, count=3)
+    got = film.sums()
+    film.close()
+    r.close()
+    assert_same_sums(got, want)
+
+
+def test_batch_of_views_matches_sequential_views(ctx, scenes):
+    cornell, cs = scenes["cornell"]
+    chart, hs = scenes["chart"]
+    turned = dataclasses.replace(chart.camera, position=(3.0, 4.0, 10.0))
+    views = [
+        (hs, chart.camera, 128, 72, 5),
+        (cs, cornell.camera, 64, 64, 7),
+        (hs, turned, 96, 54, 3),
+        (cs, dataclasses.replace(cornell.camera, vfov_degrees=30.0), 48, 80, 1),
+    ]
+    bounces = 10
+    want = [sequential_sums(ctx, s, cam, w, h, n, bounces) for s, cam, w, h, n in views]
+    # Pool capacities that force one wave for everything, several mixed
+    # waves, and waves that split one view's samples.
+    for capacity in (1 << 20, 20000, 7000):
+        r = cn.Renderer(ctx, capacity=capacity, max_bounces=bounces)
+        films = [cn.Film(ctx, w, h) for _, _, w, h, _ in views]
+        r.render_batch([(s, f, cam, n) for (s, cam, _, _, n), f in zip(views, films)])
+        for f, (_, _, _, _, n), w in zip(films, views, want):
+            assert f.samples == n
+            assert_same_sums(f.sums(), w)
+            f.close()
+        r.close()
+    cs.set_camera(cornell.camera)
+    hs.set_camera(chart.camera)
+
+
+def test_batch_continues_a_progressive_film(ctx, scenes):
+    """A batch after plain accumulates continues at the film's sample index."""
+    data, scene = scenes["cornell"]
+    want = sequential_sums(ctx, scene, data.camera, 64, 64, 6, 8)
+    scene.set_camera(data.camera)
+    r = cn.Renderer(ctx, max_bounces=8)
+    film = cn.Film(ctx, 64, 64)
+    r.accumulate(scene, film, count=2)
+    r.render_batch([(scene, film, data.camera, 4)])
+    assert film.samples == 6
+    assert_same_sums(film.sums(), want)
+    film.close()
+    r.close()
+
+
+def test_batch_rejects_a_film_twice_in_one_wave(ctx, scenes):
+    data, scene = scenes["cornell"]
+    r = cn.Renderer(ctx, max_bounces=4)
+    film = cn.Film(ctx, 16, 16)
+    with pytest.raises(cn.CenoteError):
+        r.render_batch([(scene, film, data.camera, 1), (scene, film, data.camera, 1)])
+    film.close()
+    r.close()
