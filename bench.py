@@ -114,6 +114,22 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
+def copy_calibration(gib: float = 1.0, copies: int = 8):
+    """A plain device-to-device copy kernel of known size, run under the same
+    PMC counters as the hot path: the ratio of FETCH_SIZE / WRITE_SIZE to the
+    copy's known bytes calibrates the counters' units."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(copies):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": copies}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +137,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--workload", default="chart", choices=["cornell", "chart", "heavy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by tile or "
               "view across GPUs, which this build does not run)", file=sys.stderr)
@@ -147,6 +168,8 @@ def main():
     film = cn.Film(ctx, width, height)
     pixels = width * height
     steps = args.steps
+    # Size the pool for the timed waves up front, so no wave allocates.
+    renderer.reserve(min(steps, renderer.samples_per_wave(width, height)) * pixels, aov=True)
 
     # Counting pass: the same waves the timed region traces, with traversal
     # counters on, for the algorithmic-bytes model.

@@ -216,6 +216,7 @@ SIGNATURES = {
     "cn_scene_env_tables": [_P, _FP, _FP, _FP],
     "cn_renderer_create": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)],
     "cn_renderer_destroy": [_P],
+    "cn_renderer_reserve": [_P, C.c_uint32, C.c_int],
     "cn_renderer_set_counters": [_P, C.c_int],
     "cn_renderer_wave_counts": [_P, C.POINTER(WaveCounts)],
     "cn_renderer_traversal_counts": [_P, C.POINTER(TraversalCounts)],

@@ -307,6 +307,14 @@ int cn_renderer_destroy(cn_renderer* r) {
     return CN_OK;
 }
 
+int cn_renderer_reserve(cn_renderer* r, uint32_t paths, int aov) {
+    if (!r) return fail(CN_E_INVALID, "cn_renderer_reserve: null renderer");
+    CN_HIP(hipSetDevice(r->ctx->device));
+    const size_t n = std::min<size_t>(paths, r->capacity);
+    if (int e = ensure_slots(r, n, aov != 0)) return e;
+    return ensure_pool(r, n);
+}
+
 int cn_renderer_set_counters(cn_renderer* r, int enabled) {
     if (!r) return fail(CN_E_INVALID, "cn_renderer_set_counters: null renderer");
     r->counting = enabled ? 1 : 0;

@@ -168,6 +168,7 @@ class Renderer:
         self.ctx = ctx
         self._lib = ctx._lib
         self.max_bounces = max_bounces
+        self.capacity = capacity or _abi.POOL_CAPACITY
         handle = C.c_void_p()
         check(self._lib.cn_renderer_create(ctx.handle, capacity, max_bounces, int(light_sampling), C.byref(handle)))
         self.handle = handle
@@ -229,6 +230,14 @@ class Renderer:
         b, a, n, d = film._buffers()
         check(self._lib.cn_resolve(self.handle, film.handle, _fp(b), _fp(a), _fp(n), _fp(d)))
         return b, a, n, d
+
+    def reserve(self, paths: int, aov: bool = True):
+        """Pre-size the pool and slot targets for waves of up to ``paths``."""
+        check(self._lib.cn_renderer_reserve(self.handle, paths, 1 if aov else 0))
+
+    def samples_per_wave(self, width: int, height: int) -> int:
+        """How many samples of a width x height view one wave carries."""
+        return max(1, self.capacity // (width * height))
 
     def set_counters(self, enabled: bool):
         check(self._lib.cn_renderer_set_counters(self.handle, 1 if enabled else 0))

@@ -271,6 +271,11 @@ int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional,
 int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
                        uint32_t light_sampling, cn_renderer** out);
 int cn_renderer_destroy(cn_renderer* renderer);
+/* Size the path pool and the wave's slot targets for waves of up to
+ * `paths` paths (clamped to the capacity), so that later waves allocate
+ * nothing. The reference allocates its whole pool up front
+ * (wavefront.rs:486-540); here the pool otherwise grows as waves need it. */
+int cn_renderer_reserve(cn_renderer* renderer, uint32_t paths, int aov);
 /* Count BVH node and triangle fetches in later waves (slower). */
 int cn_renderer_set_counters(cn_renderer* renderer, int enabled);
 int cn_renderer_wave_counts(cn_renderer* renderer, cn_wave_counts* out);
