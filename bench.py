@@ -35,10 +35,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {
     "camera_path": 120,   # raygen 56 + zero-fill 16 + accumulate 48
     "camera_path_aov": 144,
-    "ray": 60,            # intersect: state read 40 + hit/queue write 20
+    "ray": 60,            # trace (intersect part): state read 40 + hit/queue write 20
     "hit": 200 + 396,     # shade_surface: state 76 + write-back 56 + shadow 68; scene records 396
     "miss": 72 + 68,      # shade_miss: state 40 + radiance RMW 32; env texels + pdf 68
-    "shadow": 96,         # trace_shadow: record 64 + radiance RMW 32
+    "shadow": 96,         # trace (shadow part): record 64 + radiance RMW 32
 }
 
 
@@ -56,11 +56,11 @@ def algorithmic_bytes(counts, traversal, scene_info, pixels, waves, aov):
     return {
         "raygen": pixels * waves * 72,
         "accumulate": pixels * waves * (BYTES["camera_path_aov"] if aov else 48),
-        "intersect": BYTES["ray"] * rays + node * traversal["ray_nodes"] + tri * traversal["ray_triangles"],
+        # intersect fused with the previous bounce's trace_shadow
+        "trace": BYTES["ray"] * rays + node * traversal["ray_nodes"] + tri * traversal["ray_triangles"]
+        + BYTES["shadow"] * shadows + node * traversal["shadow_nodes"] + tri * traversal["shadow_triangles"],
         "shade_surface": BYTES["hit"] * hits,
         "shade_miss": BYTES["miss"] * misses,
-        "trace_shadow": BYTES["shadow"] * shadows + node * traversal["shadow_nodes"]
-        + tri * traversal["shadow_triangles"],
     }, dict(rays=rays, hits=hits, misses=misses, shadows=shadows)
 
 
@@ -130,12 +130,136 @@ def copy_calibration(gib: float = 1.0, copies: int = 8):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": copies}), flush=True)
 
 
+def _scenes(ctx, names):
+    import cenote_amd as cn
+    from cenote_amd import workloads
+
+    data = {n: workloads.build(n) for n in names}
+    return data, {n: cn.Scene(ctx, d) for n, d in data.items()}
+
+
+def _common(metric_value, unit, steps, warmup, seconds, config):
+    return {
+        "metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(seconds / max(1, steps) * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
+    }
+
+
+def batched_bench(args):
+    """64 seeded views of the chart and heavy scenes (128^2..1024^2, aspect
+    1..16/9, 1..64 spp): the plain form — view after view, one sample per
+    wave from the reference's 2^20-path pool, a synchronous wave each (the
+    reference's Renderer::accumulate loop) — against one many-view batch
+    (shared waves from the 2^24-path pool, launch shape from the
+    autotuner). Both forms render bitwise the same films."""
+    import cenote_amd as cn
+    from cenote_amd import autotune, workloads
+
+    ctx = cn.Context(0)
+    data, scenes = _scenes(ctx, ("chart", "heavy"))
+    views = workloads.batched_views({n: d.camera for n, d in data.items()})
+    total_paths = sum(w * h * spp for _, w, h, spp, _, _ in views)
+    log(f"batched: {len(views)} views, {total_paths / 1e6:.1f} M camera paths")
+
+    # Plain: one view at a time, one sample per synchronous wave.
+    plain_r = {d: cn.Renderer(ctx, capacity=cn.DEFAULT_CAPACITY, max_bounces=d) for d in (12, 16)}
+    films = [cn.Film(ctx, w, h) for _, w, h, _, _, _ in views]
+    warm = cn.Film(ctx, 64, 64)
+    for name, depth in (("chart", 12), ("heavy", 16)):  # first launches, outside the timing
+        plain_r[depth].accumulate(scenes[name], warm)
+    t0 = time.perf_counter()
+    for (name, w, h, spp, depth, cam), film in zip(views, films):
+        film.reset()
+        scenes[name].set_camera(cam)
+        for _ in range(spp):
+            plain_r[depth].accumulate(scenes[name], film)
+    plain_s = time.perf_counter() - t0
+    plain_sums = [f.sums()[0] for f in films]
+    for r in plain_r.values():
+        r.close()
+    log(f"plain: {plain_s:.3f} s, {total_paths / plain_s / 1e6:.1f} Mpaths/s")
+
+    # Batched: autotune on a representative view, then one render_batch.
+    r = cn.Renderer(ctx, max_bounces=16)
+    tuned = autotune.tune(r, scenes["heavy"], 1024, 576, samples=4, repeats=1)
+    log(f"autotuned shape: {tuned['shape']}")
+    r.reserve(r.capacity, aov=True)
+    warm.reset()
+    r.render_batch([(scenes["chart"], warm, data["chart"].camera, 1, 12)])
+    for f in films:
+        f.reset()
+    r.synchronize()
+    t0 = time.perf_counter()
+    r.render_batch([(scenes[n], f, cam, spp, d) for (n, w, h, spp, d, cam), f in zip(views, films)])
+    batched_s = time.perf_counter() - t0
+    same = all(np.array_equal(f.sums()[0], p, equal_nan=True) for f, p in zip(films, plain_sums))
+    log(f"batched: {batched_s:.3f} s, {total_paths / batched_s / 1e6:.1f} Mpaths/s; films identical: {same}")
+    for f in films + [warm]:
+        f.close()
+    r.close()
+    value = total_paths / batched_s / 1e6
+    out = _common(round(value, 3), "Mpaths/s", 1, 1, batched_s, {
+        "workload": f"batched: {len(views)} views of chart (depth 12) and heavy (depth 16), 128^2..1024^2, 1..64 spp",
+        "paths": total_paths, "form": "one many-view batch vs the plain one-sample-per-wave loop",
+        "shape": tuned["shape"]})
+    out["plain"] = {"value": round(total_paths / plain_s / 1e6, 3), "unit": "Mpaths/s", "seconds": round(plain_s, 3)}
+    out["speedup_vs_plain"] = round(plain_s / batched_s, 3)
+    out["films_bitwise_equal"] = bool(same)
+    return out
+
+
+def serving_bench(args):
+    """An open-loop Poisson stream of progressive requests over the resident
+    Cornell, chart and heavy scenes (256^2 / 720p / 1080p; half target 1..256
+    spp, half relMSE 1e-2 capped at 256 spp), served by the library's request
+    loop, then replayed through the plain form with the samples each request
+    needed — the same images, so the comparison is at matched noise."""
+    import cenote_amd as cn
+    from cenote_amd import serve, workloads
+
+    ctx = cn.Context(0)
+    data, scenes = _scenes(ctx, ("cornell", "chart", "heavy"))
+    depths = {n: workloads.CONFIGS[n]["max_bounces"] for n in data}
+    specs = serve.poisson_requests({n: d.camera for n, d in data.items()}, depths, args.rate, args.requests)
+    log(f"serving: {len(specs)} requests at {args.rate} req/s over {specs[-1].arrival:.2f} s")
+    r = cn.Renderer(ctx, max_bounces=16)
+    r.reserve(r.capacity, aov=True)
+    budget = 8 << 30
+    sv = serve.Server(r, budget)
+    # Warm: one small request through every scene.
+    for n in scenes:
+        sv.submit(scenes[n], data[n].camera, 64, 64, 1, max_bounces=depths[n])
+    sv.drain()
+    sv.close()
+    sv = serve.Server(r, budget)
+    served = serve.run_served(sv, scenes, specs)
+    sv.close()
+    r.close()
+    s_sum = serve.summarize(served)
+    log(f"served: {s_sum}")
+    samples = [rec["samples"] for rec in served]
+    plain = serve.run_plain(ctx, scenes, specs, samples)
+    p_sum = serve.summarize(plain)
+    log(f"plain: {p_sum}")
+    out = _common(s_sum["mpaths_per_s"], "Mpaths/s", len(specs), 0, s_sum["seconds"], {
+        "workload": f"serving: {len(specs)} Poisson requests at {args.rate}/s over cornell/chart/heavy, "
+                    "256^2/720p/1080p, 1..256 spp or relMSE 1e-2",
+        "hbm_budget_bytes": budget, "form": "continuous batching at sample granularity vs plain FIFO"})
+    out["served"] = s_sum
+    out["plain"] = p_sum
+    out["speedup_vs_plain"] = round(s_sum["mpaths_per_s"] / p_sum["mpaths_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="chart", choices=["cornell", "chart", "heavy"])
+    ap.add_argument("--workload", default="chart", choices=["cornell", "chart", "heavy", "batched", "serving"])
+    ap.add_argument("--rate", type=float, default=10.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--requests", type=int, default=48, help="serving: requests in the stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
@@ -147,6 +271,13 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by tile or "
               "view across GPUs, which this build does not run)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        print(json.dumps(batched_bench(args)), flush=True)
+        return
+    if args.workload == "serving":
+        print(json.dumps(serving_bench(args)), flush=True)
+        return
 
     import cenote_amd as cn
     from cenote_amd import workloads

@@ -29,6 +29,8 @@ LIGHT_DISTANT = 1
 LIGHT_POINT = 2
 
 TEXTURE_NONE = 0xFFFFFFFF
+BVH_DEVICE = 0
+BVH_HOST = 1
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
 POOL_CAPACITY = 1 << 24
@@ -118,6 +120,8 @@ class SceneDesc(C.Structure):
         ("delta_light_count", C.c_uint32),
         ("environment", Environment),
         ("camera", Camera),
+        ("bvh_builder", C.c_uint32),
+        ("pad", C.c_uint32),
     ]
 
 
@@ -151,7 +155,7 @@ class SceneInfo(C.Structure):
         ("bvh_node_bytes", C.c_uint32),
         ("bvh_triangle_bytes", C.c_uint32),
         ("bvh_max_depth", C.c_uint32),
-        ("pad", C.c_uint32),
+        ("bvh_builder", C.c_uint32),
         ("bvh_build_seconds", C.c_double),
         ("prep_seconds", C.c_double),
     ]
@@ -176,13 +180,73 @@ class TraversalCounts(C.Structure):
     ]
 
 
+SHADE_1024_LDS = 0
+SHADE_512_LDS = 1
+SHADE_256_LDS = 2
+SHADE_256_GLOBAL = 3
+
+
+class LaunchShape(C.Structure):
+    _fields_ = [
+        ("trace_blocks", C.c_uint32),
+        ("shade_variant", C.c_uint32),
+        ("shade_blocks", C.c_uint32),
+        ("light_blocks", C.c_uint32),
+    ]
+
+
 class ViewJob(C.Structure):
     _fields_ = [
         ("scene", C.c_void_p),
         ("film", C.c_void_p),
         ("camera", Camera),
         ("samples", C.c_uint32),
-        ("pad", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+    ]
+
+
+REQUEST_QUEUED = 0
+REQUEST_ACTIVE = 1
+REQUEST_DONE = 2
+REQUEST_RELEASED = 3
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("camera", Camera),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+        ("rel_mse", C.c_float),
+        ("pad", C.c_float),
+        ("arrival_seconds", C.c_double),
+    ]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("state", C.c_uint32),
+        ("samples", C.c_uint32),
+        ("arrival_seconds", C.c_double),
+        ("admitted_seconds", C.c_double),
+        ("first_frame_seconds", C.c_double),
+        ("done_seconds", C.c_double),
+        ("rel_mse_estimate", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("waves", C.c_uint64),
+        ("paths", C.c_uint64),
+        ("queued", C.c_uint32),
+        ("active", C.c_uint32),
+        ("hbm_used_bytes", C.c_uint64),
     ]
 
 
@@ -212,11 +276,16 @@ SIGNATURES = {
     "cn_scene_destroy": [_P],
     "cn_scene_set_camera": [_P, C.POINTER(Camera)],
     "cn_scene_info_get": [_P, C.POINTER(SceneInfo)],
+    "cn_scene_bvh": [_P, _P, C.c_uint32, _UP, _P, C.c_uint32, _UP],
     "cn_scene_light_records": [_P, C.POINTER(LightRecord), C.c_uint32, _UP],
     "cn_scene_env_tables": [_P, _FP, _FP, _FP],
     "cn_renderer_create": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)],
     "cn_renderer_destroy": [_P],
     "cn_renderer_reserve": [_P, C.c_uint32, C.c_int],
+    "cn_renderer_get_shape": [_P, C.POINTER(LaunchShape)],
+    "cn_renderer_set_shape": [_P, C.POINTER(LaunchShape)],
+    "cn_renderer_set_graphs": [_P, C.c_int],
+    "cn_renderer_graph_stats": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "cn_renderer_set_counters": [_P, C.c_int],
     "cn_renderer_wave_counts": [_P, C.POINTER(WaveCounts)],
     "cn_renderer_traversal_counts": [_P, C.POINTER(TraversalCounts)],
@@ -232,6 +301,15 @@ SIGNATURES = {
     "cn_accumulate": [_P, _P, _P],
     "cn_accumulate_async": [_P, _P, _P, C.c_uint32],
     "cn_render_batch": [_P, C.POINTER(ViewJob), C.c_uint32],
+    "cn_server_create": [_P, C.c_uint64, C.c_uint32, C.POINTER(_P)],
+    "cn_server_destroy": [_P],
+    "cn_server_clock": [_P, C.POINTER(C.c_double)],
+    "cn_server_submit": [_P, C.POINTER(RequestDesc), C.POINTER(C.c_uint64)],
+    "cn_server_step": [_P, _UP],
+    "cn_server_drain": [_P],
+    "cn_server_status": [_P, C.c_uint64, C.POINTER(RequestStatus)],
+    "cn_server_stats_get": [_P, C.POINTER(ServerStats)],
+    "cn_server_result": [_P, C.c_uint64, _FP, _FP, _FP, _FP],
     "cn_renderer_synchronize": [_P],
     "cn_film_averages": [_P, _FP, _FP, _FP, _FP],
     "cn_film_sums": [_P, _FP, _FP, _FP, _FP],

@@ -1,0 +1,79 @@
+"""Launch-shape autotuner for the wavefront stages.
+
+The stage kernels' launch shape — the persistent trace grid, the
+shade_surface workgroup variant (1024/512/256 lanes with the BSDF tables in
+LDS, or 256 lanes reading them through the caches) and the shading grid
+caps — does not change any result: every shape renders bitwise the same
+film. It does change speed, and the best shape depends on the scene (BVH
+depth, material mix) and on how many paths a wave holds. ``tune`` times a
+candidate grid on the caller's own workload and leaves the renderer on the
+fastest shape. The reference has no counterpart (its dispatch sizes come
+from its queue headers, wavefront.rs:292-356).
+"""
+
+from __future__ import annotations
+
+import itertools
+import time
+
+from . import _abi
+from .render import Film, Renderer, Scene
+
+SHADE_VARIANTS = {
+    _abi.SHADE_1024_LDS: 1024,
+    _abi.SHADE_512_LDS: 512,
+    _abi.SHADE_256_LDS: 256,
+    _abi.SHADE_256_GLOBAL: 256,
+}
+
+
+def candidate_shapes(compute_units: int, trace_per_cu=(8, 16, 32), variants=tuple(SHADE_VARIANTS)):
+    """The default search grid: trace workgroups per CU x shade variants,
+    with each shade variant's grid capped at 2048 lanes per CU."""
+    for tpc, variant in itertools.product(trace_per_cu, variants):
+        block = SHADE_VARIANTS[variant]
+        yield {
+            "trace_blocks": compute_units * tpc,
+            "shade_variant": variant,
+            "shade_blocks": compute_units * (2048 // block),
+            "light_blocks": compute_units * 4,
+        }
+
+
+def time_shape(renderer: Renderer, scene: Scene, film: Film, samples: int, repeats: int) -> float:
+    """Best wall time of ``repeats`` stream-ordered renders of ``samples``
+    samples, after one untimed render in the same shape."""
+    renderer.accumulate(scene, film, count=samples)
+    best = float("inf")
+    for _ in range(repeats):
+        film.reset()
+        renderer.synchronize()
+        t0 = time.perf_counter()
+        renderer.accumulate(scene, film, count=samples, synchronize=False)
+        renderer.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune(renderer: Renderer, scene: Scene, width: int, height: int, samples: int = 4, repeats: int = 2,
+         candidates=None) -> dict:
+    """Time every candidate shape on ``samples`` samples of a width x height
+    view of ``scene`` and set the fastest on ``renderer``. Returns the chosen
+    shape and the per-candidate timings (seconds)."""
+    _, cus = renderer.ctx.describe()
+    shapes = list(candidates if candidates is not None else candidate_shapes(cus))
+    if not shapes:
+        raise ValueError("no candidate launch shapes")
+    film = Film(renderer.ctx, width, height)
+    renderer.reserve(min(samples, renderer.samples_per_wave(width, height)) * width * height, aov=True)
+    timings = []
+    try:
+        for shape in shapes:
+            renderer.shape = shape
+            timings.append((time_shape(renderer, scene, film, samples, repeats), shape))
+    finally:
+        film.close()
+    best_time, best = min(timings, key=lambda t: t[0])
+    renderer.shape = best
+    return {"shape": best, "seconds": best_time,
+            "timings": [{"seconds": t, **shape} for t, shape in timings]}

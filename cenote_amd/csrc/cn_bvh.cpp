@@ -2,6 +2,8 @@
 
 #include "cn_bvh.h"
 
+#include "cn_bvh_split.h"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -19,41 +21,14 @@ float bits_float(uint32_t u) {
     return f;
 }
 
-struct Box {
-    float lo[3] = {INFINITY, INFINITY, INFINITY};
-    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    void grow(const Box& b) {
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::fmin(lo[k], b.lo[k]);
-            hi[k] = std::fmax(hi[k], b.hi[k]);
-        }
-    }
-    void grow(const float p[3]) {
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::fmin(lo[k], p[k]);
-            hi[k] = std::fmax(hi[k], p[k]);
-        }
-    }
-    float area() const {
-        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-        if (!(dx >= 0.0f) || !(dy >= 0.0f) || !(dz >= 0.0f)) return 0.0f;
-        return 2.0f * (dx * dy + dy * dz + dz * dx);
-    }
-};
+using Box = Box3;
 
 struct Prim {
     Box box;
     float c[3];
 };
 
-struct BNode {
-    Box box;
-    uint32_t left = 0, right = 0;  // children (internal)
-    uint32_t first = 0, count = 0;  // leaf range (count > 0)
-};
-
-constexpr int BINS = 32;
-constexpr uint32_t LEAF_SIZE = 4;
+using BNode = BinaryNode;
 
 struct Builder {
     const std::vector<Prim>& prims;
@@ -68,84 +43,49 @@ struct Builder {
 
     void build(uint32_t node, uint32_t begin, uint32_t end, int depth) {
         BNode& out = nodes[node];
-        Box box, cbox;
+        Box box = Box::empty(), cbox = Box::empty();
         for (uint32_t i = begin; i < end; ++i) {
             const Prim& p = prims[order[i]];
             box.grow(p.box);
-            cbox.grow(p.c);
+            for (int k = 0; k < 3; ++k) {
+                cbox.lo[k] = std::fmin(cbox.lo[k], p.c[k]);
+                cbox.hi[k] = std::fmax(cbox.hi[k], p.c[k]);
+            }
         }
         out.box = box;
         const uint32_t n = end - begin;
-        if (n <= 1) {
-            out.first = begin;
-            out.count = n;
-            return;
-        }
-        // Binned SAH over all three axes.
-        float best_cost = INFINITY;
-        int best_axis = -1, best_split = 0;
+        Box bins[3][BVH_BINS];
+        uint32_t counts[3][BVH_BINS] = {};
         for (int axis = 0; axis < 3; ++axis) {
-            const float extent = cbox.hi[axis] - cbox.lo[axis];
-            if (!(extent > 0.0f)) continue;
-            const float scale = float(BINS) / extent;
-            Box bins[BINS];
-            uint32_t counts[BINS] = {};
+            for (int b = 0; b < BVH_BINS; ++b) bins[axis][b] = Box::empty();
+            if (n <= 1 || !(cbox.hi[axis] - cbox.lo[axis] > 0.0f)) continue;
+            const float scale = bvh_scale(cbox.lo[axis], cbox.hi[axis]);
             for (uint32_t i = begin; i < end; ++i) {
                 const Prim& p = prims[order[i]];
-                int b = int((p.c[axis] - cbox.lo[axis]) * scale);
-                b = std::min(std::max(b, 0), BINS - 1);
-                bins[b].grow(p.box);
-                ++counts[b];
-            }
-            float right_area[BINS];
-            uint32_t right_count[BINS];
-            Box acc;
-            uint32_t cnt = 0;
-            for (int b = BINS - 1; b > 0; --b) {
-                acc.grow(bins[b]);
-                cnt += counts[b];
-                right_area[b] = acc.area();
-                right_count[b] = cnt;
-            }
-            Box lacc;
-            uint32_t lcnt = 0;
-            for (int b = 0; b < BINS - 1; ++b) {
-                lacc.grow(bins[b]);
-                lcnt += counts[b];
-                if (lcnt == 0 || right_count[b + 1] == 0) continue;
-                const float cost = lacc.area() * float(lcnt) + right_area[b + 1] * float(right_count[b + 1]);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_axis = axis;
-                    best_split = b;
-                }
+                const int b = bvh_bin(p.c[axis], cbox.lo[axis], scale);
+                bins[axis][b].grow(p.box);
+                ++counts[axis][b];
             }
         }
-        const float parent_area = box.area();
-        const float leaf_cost = float(n);
-        const float split_cost = 1.0f + (parent_area > 0.0f ? best_cost / parent_area : 0.0f);
-        if (n <= LEAF_SIZE && (best_axis < 0 || leaf_cost <= split_cost)) {
+        const SplitDecision d = bvh_decide(box, cbox, bins, counts, n);
+        if (d.leaf) {
             out.first = begin;
             out.count = n;
             return;
         }
-        uint32_t mid;
-        if (best_axis < 0) {
-            // All centroids coincide: split the range in half.
-            mid = begin + n / 2;
-        } else {
-            const float extent = cbox.hi[best_axis] - cbox.lo[best_axis];
-            const float scale = float(BINS) / extent;
-            const float lo = cbox.lo[best_axis];
-            auto* first = order.data() + begin;
-            auto* last = order.data() + end;
-            auto* m = std::partition(first, last, [&](uint32_t id) {
-                int b = int((prims[id].c[best_axis] - lo) * scale);
-                b = std::min(std::max(b, 0), BINS - 1);
-                return b <= best_split;
-            });
-            mid = uint32_t(m - order.data());
-            if (mid == begin || mid == end) mid = begin + n / 2;
+        // Stable partition, so that the device builder (which scatters with
+        // prefix sums) orders every range the same way.
+        uint32_t mid = begin + n / 2;
+        if (d.axis >= 0) {
+            const uint32_t left = bvh_left_count(d, counts);
+            if (left > 0 && left < n) {
+                const float lo = cbox.lo[d.axis];
+                const float scale = bvh_scale(cbox.lo[d.axis], cbox.hi[d.axis]);
+                std::stable_partition(order.data() + begin, order.data() + end, [&](uint32_t id) {
+                    return bvh_bin(prims[id].c[d.axis], lo, scale) <= d.split;
+                });
+                mid = begin + left;
+            }
         }
         const uint32_t children = alloc2();
         out.left = children;
@@ -238,6 +178,22 @@ struct Collapser {
 
 }  // namespace
 
+bool collapse_to_bvh4(const std::vector<BinaryNode>& bnodes, float max_abs, uint32_t max_stack, Bvh4* out,
+                      std::string* error) {
+    // Boxes grow by an absolute margin proportional to the scene's extent so
+    // the traversal's fused slab test stays conservative.
+    Collapser col{bnodes, out->nodes, 4e-6f * std::fmax(max_abs, 1e-3f)};
+    out->nodes.clear();
+    out->nodes.reserve(bnodes.size() / 2 + 1);
+    col.collapse(0, 1);
+    out->max_depth = col.max_depth;
+    if (3 * col.max_depth > max_stack) {
+        *error = "BVH too deep for the traversal stack (depth " + std::to_string(col.max_depth) + ")";
+        return false;
+    }
+    return true;
+}
+
 bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh4* out, std::string* error) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = uint32_t(input.size());
@@ -251,17 +207,19 @@ bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh
     }
     std::vector<Prim> prims(n);
     float max_abs = 0.0f;
+    bool finite = true;
     for (uint32_t i = 0; i < n; ++i) {
         const WorldTriangle& t = input[i];
         const float p[3][3] = {{t.v0.x, t.v0.y, t.v0.z}, {t.v1.x, t.v1.y, t.v1.z}, {t.v2.x, t.v2.y, t.v2.z}};
         Prim& pr = prims[i];
-        for (const auto& q : p) {
-            pr.box.grow(q);
-            for (int k = 0; k < 3; ++k) max_abs = std::fmax(max_abs, std::fabs(q[k]));
-        }
-        for (int k = 0; k < 3; ++k) pr.c[k] = 0.5f * (pr.box.lo[k] + pr.box.hi[k]);
+        bvh_prim_bounds(p, &pr.box, pr.c);
+        for (const auto& q : p)
+            for (int k = 0; k < 3; ++k) {
+                max_abs = std::fmax(max_abs, std::fabs(q[k]));
+                finite = finite && std::isfinite(q[k]);
+            }
     }
-    if (!std::isfinite(max_abs)) {
+    if (!finite) {
         *error = "scene geometry holds a non-finite vertex";
         return false;
     }
@@ -271,18 +229,7 @@ bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh
     Builder builder(prims, order, bnodes);
     builder.build(0, 0, n, 0);
     bnodes.resize(builder.next.load());
-
-    // Boxes grow by an absolute margin proportional to the scene's extent so
-    // the traversal's fused slab test stays conservative.
-    Collapser col{bnodes, out->nodes, 4e-6f * std::fmax(max_abs, 1e-3f)};
-    out->nodes.clear();
-    out->nodes.reserve(bnodes.size() / 2 + 1);
-    col.collapse(0, 1);
-    out->max_depth = col.max_depth;
-    if (3 * col.max_depth > max_stack) {
-        *error = "BVH too deep for the traversal stack (depth " + std::to_string(col.max_depth) + ")";
-        return false;
-    }
+    if (!collapse_to_bvh4(bnodes, max_abs, max_stack, out, error)) return false;
 
     out->tris.resize(n);
     for (uint32_t i = 0; i < n; ++i) {

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "cn_bvh_split.h"
 #include "cn_math.h"
 #include "cn_records.h"
 
@@ -24,10 +25,32 @@ struct Bvh4 {
     double build_seconds = 0.0;
 };
 
+// A node of the binary tree both builders grow (40 bytes, the layout the
+// device builder writes).
+struct BinaryNode {
+    Box3 box;
+    uint32_t left = 0, right = 0;   // children (internal)
+    uint32_t first = 0, count = 0;  // leaf range in the builder's order (count > 0)
+};
+static_assert(sizeof(BinaryNode) == 40, "device and host share this layout");
+
 // Binned-SAH binary build (32 bins on the three axes, leaves of <= 4
 // triangles), collapsed to 4-wide nodes by repeatedly opening the child of
 // largest surface area. Returns false with a message when the tree would
 // be deeper than the traversal stack allows.
 bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh4* out, std::string* error);
+
+// The collapse step alone, from a binary tree rooted at node 0 (nodes may
+// be numbered in any order); max_abs is the scene's largest |coordinate|.
+bool collapse_to_bvh4(const std::vector<BinaryNode>& bnodes, float max_abs, uint32_t max_stack, Bvh4* out,
+                      std::string* error);
+
+// The same tree built by device passes (cn_build.hip): top levels by
+// device-wide binning and stable scatter with the split decisions made on
+// the host, subtrees of <= 2048 triangles by one workgroup each; collapsed
+// on the host. Leaf-ordered triangles stay on the device: `*tris` is a
+// device allocation the caller frees with hipFree.
+bool build_bvh4_device(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh4* out, BvhTriangle** tris,
+                       std::string* error);
 
 }  // namespace cn

@@ -66,16 +66,7 @@ EventProfiler::~EventProfiler() {
 
 }  // namespace cn
 
-namespace {
-
-// One job as the host schedules it.
-struct HostJob {
-    cn_scene* scene = nullptr;
-    CameraView cam{};
-    uint32_t width = 0, height = 0;
-    uint32_t first_sample = 0, samples = 0;
-    cn_film* film = nullptr;  // null: trace only, no accumulate
-};
+namespace cn {
 
 // The path pool grows on demand up to the renderer's capacity, so a large
 // capacity costs HBM only when views large enough to use it arrive.
@@ -83,6 +74,7 @@ int ensure_pool(cn_renderer* r, size_t paths) {
     paths = std::min<size_t>(paths, r->capacity);
     if (r->origin.count >= paths) return CN_OK;
     CN_HIP(hipStreamSynchronize(r->stream));
+    r->drop_graphs();
     CN_HIP(r->origin.alloc(paths));
     CN_HIP(r->direction.alloc(paths));
     CN_HIP(r->throughput.alloc(paths));
@@ -119,6 +111,7 @@ int ensure_slots(cn_renderer* r, size_t slots, bool aov) {
     const bool grow = r->radiance.count < slots || (aov && r->albedo.count < slots);
     if (!grow) return CN_OK;
     CN_HIP(hipStreamSynchronize(r->stream));  // nothing in flight may still use the old buffers
+    r->drop_graphs();
     const size_t n = std::max(slots, r->radiance.count);
     if (r->radiance.count < n) CN_HIP(r->radiance.alloc(n));
     if (aov) {
@@ -130,10 +123,14 @@ int ensure_slots(cn_renderer* r, size_t slots, bool aov) {
     return CN_OK;
 }
 
-// Enqueue one wave on the renderer's stream.
 int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
     if (jobs.empty()) return CN_OK;
     if (jobs.size() > MAX_JOBS) return fail(CN_E_INVALID, "a wave holds at most %u jobs", MAX_JOBS);
+    for (HostJob& j : jobs) {
+        if (j.max_bounces == 0) j.max_bounces = r->max_bounces;
+        if (j.max_bounces > r->max_bounces)
+            return fail(CN_E_INVALID, "a view's depth %u exceeds the renderer's %u", j.max_bounces, r->max_bounces);
+    }
     // Group by scene, keeping first-appearance order (a scene's group walks
     // one contiguous slot span, so one SceneView serves each range).
     {
@@ -168,7 +165,11 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
     r->ring_next = (r->ring_next + 1) % JOB_RING;
     if (r->ring_used[e]) CN_HIP(hipEventSynchronize(r->ring_event[e]));
     JobRecord* host = r->staging + size_t(e) * MAX_JOBS;
-    JobRecord* dev = r->jobs.ptr + size_t(e) * MAX_JOBS;
+    // With graphs, every wave reads its job table from one fixed device
+    // table (stream order keeps a wave's table until its kernels are done),
+    // so a captured launch sequence replays for any wave of the same shape.
+    const bool graph = r->graphs && !r->profiler && !r->counting;
+    JobRecord* dev = r->jobs.ptr + (graph ? 0 : size_t(e) * MAX_JOBS);
     uint32_t slot_base = 0, pixel_base = 0;
     for (size_t i = 0; i < jobs.size(); ++i) {
         const HostJob& j = jobs[i];
@@ -183,11 +184,13 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
         rec.first_sample = j.first_sample;
         rec.samples = j.samples;
         rec.flags = (j.film && j.first_sample == 0) ? JOB_RESET : 0u;
+        rec.max_bounces = j.max_bounces;
         if (j.film) {
             rec.beauty_sum = j.film->beauty_sum.ptr;
             rec.albedo_sum = j.film->albedo_sum.ptr;
             rec.normal_sum = j.film->normal_sum.ptr;
             rec.depth_sum = j.film->depth_sum.ptr;
+            rec.moment_sum = j.moments ? j.film->moment_sum.ptr : nullptr;
         }
         slot_base += rec.pixels * rec.samples;
         pixel_base += rec.pixels;
@@ -202,35 +205,96 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
         tg.depth = r->depth.ptr;
         tg.guide = r->guide.ptr;
     }
-    const uint32_t per = counters_per_range(r->max_bounces);
-    uint32_t range = 0;
+    bool films = false;
+    for (const HostJob& j : jobs) films = films || j.film != nullptr;
+
+    // The wave's launch plan: per scene group, pool-sized ranges of
+    // (base, count, bounce rounds).
+    struct Range {
+        size_t g0, g1;
+        uint32_t base, count, rounds;
+    };
+    std::vector<Range> plan;
     size_t g0 = 0;
     while (g0 < jobs.size()) {
         size_t g1 = g0;
         while (g1 < jobs.size() && jobs[g1].scene == jobs[g0].scene) ++g1;
+        uint32_t rounds = 0;  // bounce rounds: the group's deepest cap
+        for (size_t k = g0; k < g1; ++k) rounds = std::max(rounds, jobs[k].max_bounces);
         const uint32_t s0 = host[g0].slot_base;
         const uint32_t s1 = host[g1 - 1].slot_base + host[g1 - 1].pixels * host[g1 - 1].samples;
-        for (uint64_t base = s0; base < s1; base += r->capacity) {
-            const uint32_t count = uint32_t(std::min<uint64_t>(r->capacity, s1 - base));
+        for (uint64_t base = s0; base < s1; base += r->capacity)
+            plan.push_back({g0, g1, uint32_t(base), uint32_t(std::min<uint64_t>(r->capacity, s1 - base)), rounds});
+        g0 = g1;
+    }
+    r->last_ranges = uint32_t(plan.size());
+
+    const uint32_t per = counters_per_range(r->max_bounces);
+    auto launch_all = [&]() -> int {
+        for (size_t k = 0; k < plan.size(); ++k) {
+            const Range& rg = plan[k];
             // Ranges run in stream order and raygen clears its counters, so
             // counter sets are reused past MAX_RANGES (only the per-wave
             // diagnostics need them distinct).
-            uint32_t* counters = r->counters.ptr + size_t(range % MAX_RANGES) * per;
-            CN_HIP(launch_raygen(r->stream, dev + g0, uint32_t(g1 - g0), r->pool, tg, counters, per, uint32_t(base),
-                                 count, r->profiler.get()));
-            for (uint32_t b = 0; b < r->max_bounces; ++b) {
-                CN_HIP(launch_bounce(r->stream, r->shape, jobs[g0].scene->view, r->pool, tg, counters,
-                                     uint32_t(base), count, b, r->max_bounces, r->light_sampling, r->counting,
-                                     r->profiler.get()));
+            uint32_t* counters = r->counters.ptr + (k % MAX_RANGES) * per;
+            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), r->pool, tg, counters, per, rg.base,
+                                 rg.count, r->profiler.get()));
+            for (uint32_t b = 0; b < rg.rounds; ++b) {
+                CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, r->pool, tg, counters, rg.base,
+                                     rg.count, b, r->light_sampling, r->counting, r->profiler.get()));
             }
-            ++range;
         }
-        g0 = g1;
+        if (films)
+            CN_HIP(launch_accumulate(r->stream, dev, uint32_t(jobs.size()), tg, uint32_t(pixels), r->profiler.get()));
+        return CN_OK;
+    };
+
+    if (!graph) {
+        if (int err = launch_all()) return err;
+    } else {
+        // Everything that reaches a kernel argument, except the job table's
+        // contents (read from device memory at run time).
+        std::vector<uint64_t> key = {uint64_t(jobs.size()), pixels, uint64_t(films), uint64_t(aov),
+                                     uint64_t(uintptr_t(tg.radiance)), uint64_t(uintptr_t(r->pool.origin)),
+                                     uint64_t(uintptr_t(r->pool.stack_overflow)), r->shape.trace_blocks,
+                                     r->shape.shade_variant, r->shape.shade_blocks, r->shape.light_blocks,
+                                     r->light_sampling};
+        for (const Range& rg : plan) {
+            key.insert(key.end(), {jobs[rg.g0].scene->serial, uint64_t(rg.g0), uint64_t(rg.g1), rg.base, rg.count,
+                                   rg.rounds});
+        }
+        WaveGraph* hit = nullptr;
+        for (WaveGraph& g : r->graph_cache)
+            if (g.key == key) hit = &g;
+        if (!hit) {
+            if (r->graph_cache.size() >= 32) {
+                auto lru = std::min_element(r->graph_cache.begin(), r->graph_cache.end(),
+                                            [](const WaveGraph& x, const WaveGraph& y) { return x.last_use < y.last_use; });
+                (void)hipGraphExecDestroy(lru->exec);
+                r->graph_cache.erase(lru);
+            }
+            hipGraph_t g = nullptr;
+            CN_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            const int err = launch_all();
+            const hipError_t end = hipStreamEndCapture(r->stream, &g);
+            if (err) {
+                if (g) (void)hipGraphDestroy(g);
+                return err;
+            }
+            CN_HIP(end);
+            hipGraphExec_t exec = nullptr;
+            const hipError_t inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CN_HIP(inst);
+            r->graph_cache.push_back({std::move(key), exec, 0});
+            hit = &r->graph_cache.back();
+            r->graph_captures += 1;
+        } else {
+            r->graph_hits += 1;
+        }
+        hit->last_use = ++r->graph_clock;
+        CN_HIP(hipGraphLaunch(hit->exec, r->stream));
     }
-    r->last_ranges = range;
-    bool films = false;
-    for (const HostJob& j : jobs) films = films || j.film != nullptr;
-    if (films) CN_HIP(launch_accumulate(r->stream, dev, uint32_t(jobs.size()), tg, uint32_t(pixels), r->profiler.get()));
     CN_HIP(hipEventRecord(r->ring_event[e], r->stream));
     r->ring_used[e] = true;
     return CN_OK;
@@ -257,7 +321,7 @@ int film_job(cn_renderer* r, cn_scene* s, cn_film* f, const cn_camera& camera, u
     return scene_camera(s, camera, f->width, f->height, &job->cam);
 }
 
-}  // namespace
+}  // namespace cn
 
 extern "C" {
 
@@ -274,8 +338,11 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     r->max_bounces = max_bounces;
     r->light_sampling = light_sampling;
     const uint32_t cus = std::max(1u, ctx->compute_units);
-    r->shape.trace_blocks = cus * 8;
-    r->shape.shade_blocks = cus * 2;
+    // Defaults from the autotuner's sweeps on the chart and heavy scenes;
+    // cn_renderer_set_shape overrides them.
+    r->shape.trace_blocks = cus * 16;
+    r->shape.shade_variant = SHADE_512_LDS;
+    r->shape.shade_blocks = cus * 4;
     r->shape.light_blocks = cus * 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * 256 * STACK_SPILL));
@@ -315,6 +382,54 @@ int cn_renderer_reserve(cn_renderer* r, uint32_t paths, int aov) {
     return ensure_pool(r, n);
 }
 
+int cn_renderer_get_shape(cn_renderer* r, cn_launch_shape* out) {
+    if (!r || !out) return fail(CN_E_INVALID, "cn_renderer_get_shape: null argument");
+    out->trace_blocks = r->shape.trace_blocks;
+    out->shade_variant = r->shape.shade_variant;
+    out->shade_blocks = r->shape.shade_blocks;
+    out->light_blocks = r->shape.light_blocks;
+    return CN_OK;
+}
+
+int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
+    if (!r || !shape) return fail(CN_E_INVALID, "cn_renderer_set_shape: null argument");
+    auto in_range = [](uint32_t v) { return v >= 1 && v <= 65536; };
+    if (!in_range(shape->trace_blocks) || !in_range(shape->shade_blocks) || !in_range(shape->light_blocks))
+        return fail(CN_E_INVALID, "launch shape block counts must lie in 1..65536");
+    if (shape->shade_variant >= SHADE_VARIANTS)
+        return fail(CN_E_INVALID, "unknown shade variant %u", shape->shade_variant);
+    CN_HIP(hipSetDevice(r->ctx->device));
+    CN_HIP(hipStreamSynchronize(r->stream));
+    // The traversal stack's spill area is per lane of the trace grid.
+    const size_t spill = size_t(shape->trace_blocks) * 256 * STACK_SPILL;
+    if (r->spill.count < spill) {
+        r->drop_graphs();
+        CN_HIP(r->spill.alloc(spill));
+        r->pool.stack_overflow = r->spill.ptr;
+    }
+    r->shape.trace_blocks = shape->trace_blocks;
+    r->shape.shade_variant = shape->shade_variant;
+    r->shape.shade_blocks = shape->shade_blocks;
+    r->shape.light_blocks = shape->light_blocks;
+    return CN_OK;
+}
+
+int cn_renderer_set_graphs(cn_renderer* r, int enabled) {
+    if (!r) return fail(CN_E_INVALID, "cn_renderer_set_graphs: null renderer");
+    CN_HIP(hipSetDevice(r->ctx->device));
+    CN_HIP(hipStreamSynchronize(r->stream));
+    r->graphs = enabled != 0;
+    if (!r->graphs) r->drop_graphs();
+    return CN_OK;
+}
+
+int cn_renderer_graph_stats(cn_renderer* r, uint64_t* captures, uint64_t* replays) {
+    if (!r || !captures || !replays) return fail(CN_E_INVALID, "cn_renderer_graph_stats: null argument");
+    *captures = r->graph_captures;
+    *replays = r->graph_hits;
+    return CN_OK;
+}
+
 int cn_renderer_set_counters(cn_renderer* r, int enabled) {
     if (!r) return fail(CN_E_INVALID, "cn_renderer_set_counters: null renderer");
     r->counting = enabled ? 1 : 0;
@@ -343,8 +458,7 @@ int cn_renderer_kernel_times(cn_renderer* r, cn_kernel_times* out) {
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
     r->profiler->collect();
-    static const char* names[K_KINDS] = {"raygen", "intersect", "shade_miss", "shade_surface", "trace_shadow",
-                                         "accumulate"};
+    static const char* names[K_KINDS] = {"raygen", "trace", "shade_miss", "shade_surface", "accumulate"};
     std::memset(out, 0, sizeof(*out));
     out->kernels = K_KINDS;
     for (int k = 0; k < K_KINDS; ++k) {
@@ -522,6 +636,7 @@ int cn_render_batch(cn_renderer* r, const cn_view_job* views, uint32_t count) {
             if (px > budget && !wave.empty()) continue;
             HostJob job;
             if (int e = film_job(r, views[i].scene, views[i].film, views[i].camera, 1, &job)) return e;
+            job.max_bounces = views[i].max_bounces;
             wave.push_back(job);
             who.push_back(i);
             budget = px > budget ? 0 : budget - px;

@@ -50,6 +50,11 @@ struct DevBuf {
         if (n == 0) return hipSuccess;
         return hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
     }
+    void adopt(T* p, size_t n) {  // take ownership of a device allocation
+        reset();
+        ptr = p;
+        count = n;
+    }
     hipError_t upload(const T* src, size_t n) {
         hipError_t e = alloc(n);
         if (e != hipSuccess || n == 0) return e;
@@ -86,6 +91,7 @@ struct cn_context {
 
 struct cn_scene {
     cn_context* ctx = nullptr;
+    uint64_t serial = 0;  // unique per scene created in this process (graph-cache keys)
     cn_camera camera{};
     cn::DevBuf<cn::BvhNode4> nodes;
     cn::DevBuf<cn::BvhTriangle> tris;
@@ -108,6 +114,7 @@ struct cn_film {
     uint32_t samples = 0;
     cn::DevBuf<float4> beauty_sum, albedo_sum, normal_sum;
     cn::DevBuf<float> depth_sum;
+    cn::DevBuf<float> moment_sum;  // squared-luminance sums, for noise targets (allocated on demand)
     // A one-sample staging set for cn_film_set_sample / cn_film_accumulate_sample.
     cn::DevBuf<float4> beauty, albedo, normal;
     cn::DevBuf<float> depth;
@@ -116,9 +123,44 @@ struct cn_film {
     cn::DevBuf<float> out_depth;
 };
 
+namespace cn {
+
+// One job of a wave as the host schedules it: `samples` consecutive samples
+// from `first_sample` of one view.
+struct HostJob {
+    cn_scene* scene = nullptr;
+    CameraView cam{};
+    uint32_t width = 0, height = 0;
+    uint32_t first_sample = 0, samples = 0;
+    uint32_t max_bounces = 0;  // the job's depth cap; 0 = the renderer's
+    cn_film* film = nullptr;   // null: trace only, no accumulate
+    bool moments = false;      // also fold squared luminance (the film must hold a moment buffer)
+};
+
+// Enqueue one wave of jobs on the renderer's stream (no host sync unless a
+// buffer must grow or a job-table ring entry is still in flight).
+int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov);
+
+// A film job for `samples` more samples of `camera` on `s`, continuing at
+// the film's sample count.
+int film_job(cn_renderer* r, cn_scene* s, cn_film* f, const cn_camera& camera, uint32_t samples, HostJob* job);
+
+// Samples of a `pixels`-pixel view that fill one wave of the renderer's pool.
+uint32_t samples_per_wave(const cn_renderer* r, uint32_t pixels, uint32_t wanted);
+
+}  // namespace cn
+
 // Job tables in flight: a ring of device tables, each filled from pinned
 // host staging and released by an event once its wave has finished.
 constexpr uint32_t JOB_RING = 32;
+
+// One captured wave: its launch sequence as an executable graph, keyed by
+// everything that reaches a kernel argument.
+struct WaveGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
 
 struct cn_renderer {
     cn_context* ctx = nullptr;
@@ -149,4 +191,16 @@ struct cn_renderer {
     uint32_t ring_next = 0;
     uint32_t last_ranges = 0;
     cn::PathPool pool{};
+    // hipGraph per wave shape: waves replay a captured launch sequence when
+    // their shape repeats (progressive frames of the same views).
+    bool graphs = false;
+    std::vector<WaveGraph> graph_cache;
+    uint64_t graph_clock = 0;
+    uint64_t graph_hits = 0, graph_captures = 0;
+    void drop_graphs() {
+        for (WaveGraph& g : graph_cache)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graph_cache.clear();
+    }
+    ~cn_renderer() { drop_graphs(); }
 };

@@ -15,12 +15,21 @@ constexpr uint32_t STACK_LDS = 16;
 constexpr uint32_t STACK_MAX = 128;
 constexpr uint32_t STACK_SPILL = STACK_MAX - STACK_LDS;
 
+// shade_surface launch variants: workgroup size and where the BSDF tables live.
+enum ShadeVariant : uint32_t { SHADE_1024_LDS = 0, SHADE_512_LDS = 1, SHADE_256_LDS = 2, SHADE_256_GLOBAL = 3,
+                               SHADE_VARIANTS = 4 };
+
+inline uint32_t shade_block(uint32_t variant) {
+    return variant == SHADE_1024_LDS ? 1024u : variant == SHADE_512_LDS ? 512u : 256u;
+}
+
 // Grid sizes: the traversal kernels run a fixed, chip-filling grid and
 // walk their queue; the shading kernels are capped likewise.
 struct LaunchShape {
-    uint32_t trace_blocks;  // blocks of 256 lanes for intersect / trace_shadow
-    uint32_t shade_blocks;  // blocks of 1024 lanes for shade_surface
-    uint32_t light_blocks;  // blocks of 256 lanes for shade_miss
+    uint32_t trace_blocks;   // blocks of 256 lanes for trace
+    uint32_t shade_variant;  // a ShadeVariant
+    uint32_t shade_blocks;   // most blocks (of the variant's size) for shade_surface
+    uint32_t light_blocks;   // blocks of 256 lanes for shade_miss
 };
 
 // The film's buffers (render/film.rs:66-127): one wave's sample
@@ -37,7 +46,8 @@ struct FilmView {
 };
 
 // The stage kernels, for per-kernel timing.
-enum KernelKind { K_RAYGEN = 0, K_INTERSECT, K_SHADE_MISS, K_SHADE_SURFACE, K_TRACE_SHADOW, K_ACCUMULATE, K_KINDS };
+// K_TRACE is intersect fused with the previous bounce's trace_shadow.
+enum KernelKind { K_RAYGEN = 0, K_TRACE, K_SHADE_MISS, K_SHADE_SURFACE, K_ACCUMULATE, K_KINDS };
 
 // Brackets each launch (the renderer records HIP events on the launch
 // stream when profiling is on).
@@ -55,8 +65,7 @@ hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, int counting,
-                         Profiler* prof);
+                         uint32_t bounce, uint32_t light_sampling, int counting, Profiler* prof);
 
 // Fold every job's slots into its film sums (jobs sorted by pixel base;
 // total_pixels is the sum of the jobs' view pixels).
@@ -66,6 +75,11 @@ hipError_t launch_accumulate(hipStream_t stream, const JobRecord* jobs, uint32_t
 // Divides the sums into the FilmView's sample-buffer slots (the caller
 // passes the resolve targets there).
 hipError_t launch_resolve(hipStream_t stream, const FilmView& f, uint32_t pixels, float samples);
+
+// Per-block partial sums of the estimated relMSE of a film's mean
+// (`blocks` doubles; samples >= 2).
+hipError_t launch_noise(hipStream_t stream, const float4* sums, const float* moments, uint32_t pixels, float samples,
+                        double* partials, uint32_t blocks);
 
 hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t dimension, uint32_t count,
                                   float* points, float* values);

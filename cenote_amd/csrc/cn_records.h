@@ -34,6 +34,9 @@ constexpr uint32_t LIGHT_KIND_TRIANGLE = 0;     // shaders/lights.slang:23-25
 constexpr uint32_t LIGHT_KIND_DISTANT = 1;
 constexpr uint32_t LIGHT_KIND_POINT = 2;
 
+// ACEScg luminance weights (color.rs:29-31).
+constexpr float LUMA_R = 0.2722287f, LUMA_G = 0.6740818f, LUMA_B = 0.0536895f;
+
 // --- BVH -------------------------------------------------------------------
 //
 // A 4-wide BVH over world-space triangles (the flattened TLAS x BLAS of
@@ -115,7 +118,7 @@ struct SceneView {
 // space; a path's target slot is its range base plus its pool index, and
 // its sampler key (pixel within its view, sample index) rides in `key`.
 struct PathPool {
-    float4* origin;       // xyz origin
+    float4* origin;       // xyz origin, w = the path's depth cap (bits)
     float4* direction;    // xyz direction, w = cosine to the camera forward
     uint4* hit;           // instance, primitive, b1 bits, b2 bits
     float4* throughput;   // xyz weight, w = previous scatter pdf
@@ -165,10 +168,12 @@ struct JobRecord {
     uint32_t slot_base, pixel_base;
     uint32_t first_sample, samples;
     uint32_t flags;
+    uint32_t max_bounces;  // the job's depth cap (paths carry it in origin.w)
     float4* beauty_sum;
     float4* albedo_sum;
     float4* normal_sum;
     float* depth_sum;
+    float* moment_sum;  // when set: the running sum of each sample's squared luminance
 };
 
 // Jobs one wave may carry.

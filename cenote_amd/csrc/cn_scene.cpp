@@ -5,6 +5,8 @@
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -130,6 +132,8 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     }
     CN_HIP(hipSetDevice(ctx->device));
     auto scene = std::make_unique<cn_scene>();
+    static std::atomic<uint64_t> serials{1};
+    scene->serial = serials.fetch_add(1);
     scene->ctx = ctx;
     scene->camera = desc->camera;
 
@@ -217,13 +221,21 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
                                                desc->environment.texels);
     const float select = select_probability(scene->host_env.power, light_power);
 
+    if (desc->bvh_builder != CN_BVH_DEVICE && desc->bvh_builder != CN_BVH_HOST)
+        return fail(CN_E_INVALID, "unknown BVH builder %u", desc->bvh_builder);
     Bvh4 bvh;
     std::string error;
-    if (!build_bvh4(world, STACK_MAX, &bvh, &error)) return fail(CN_E_SCENE, "%s", error.c_str());
+    if (desc->bvh_builder == CN_BVH_HOST) {
+        if (!build_bvh4(world, STACK_MAX, &bvh, &error)) return fail(CN_E_SCENE, "%s", error.c_str());
+        CN_HIP(scene->tris.upload(bvh.tris.data(), bvh.tris.size()));
+    } else {
+        BvhTriangle* tris = nullptr;
+        if (!build_bvh4_device(world, STACK_MAX, &bvh, &tris, &error)) return fail(CN_E_SCENE, "%s", error.c_str());
+        scene->tris.adopt(tris, world.size());
+    }
 
     // Upload.
     CN_HIP(scene->nodes.upload(bvh.nodes.data(), bvh.nodes.size()));
-    CN_HIP(scene->tris.upload(bvh.tris.data(), bvh.tris.size()));
     CN_HIP(scene->instances.upload(records.data(), records.size()));
     CN_HIP(scene->materials.upload(materials.data(), materials.size()));
     if (scene->host_lights.empty()) {
@@ -273,6 +285,7 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     info.bvh_node_bytes = sizeof(BvhNode4);
     info.bvh_triangle_bytes = sizeof(BvhTriangle);
     info.bvh_max_depth = bvh.max_depth;
+    info.bvh_builder = desc->bvh_builder;
     info.bvh_build_seconds = bvh.build_seconds;
     info.prep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = scene.release();
@@ -283,6 +296,23 @@ int cn_scene_destroy(cn_scene* scene) {
     if (!scene) return fail(CN_E_INVALID, "cn_scene_destroy: null scene");
     (void)hipSetDevice(scene->ctx->device);
     delete scene;
+    return CN_OK;
+}
+
+int cn_scene_bvh(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t* node_count, void* triangles,
+                 uint32_t triangle_capacity, uint32_t* triangle_count) {
+    if (!scene) return fail(CN_E_INVALID, "cn_scene_bvh: null scene");
+    CN_HIP(hipSetDevice(scene->ctx->device));
+    if (node_count) *node_count = uint32_t(scene->nodes.count);
+    if (triangle_count) *triangle_count = uint32_t(scene->tris.count);
+    if (nodes) {
+        if (node_capacity < scene->nodes.count) return fail(CN_E_INVALID, "node buffer too small");
+        CN_HIP(hipMemcpy(nodes, scene->nodes.ptr, scene->nodes.count * sizeof(BvhNode4), hipMemcpyDeviceToHost));
+    }
+    if (triangles) {
+        if (triangle_capacity < scene->tris.count) return fail(CN_E_INVALID, "triangle buffer too small");
+        CN_HIP(hipMemcpy(triangles, scene->tris.ptr, scene->tris.count * sizeof(BvhTriangle), hipMemcpyDeviceToHost));
+    }
     return CN_OK;
 }
 

@@ -371,10 +371,15 @@ __device__ inline Closure resolve_closure(const float* t, const Mat& m, v3 wo, b
 
 // openpbr.slang:596-608
 __device__ __forceinline__ float aov_record_fraction(const Closure& c, uint32_t lobe) {
+    // The three candidates are read unconditionally and pinned in registers:
+    // a branch per field lets the optimiser merge the loads into one load at
+    // a lobe-dependent offset, which forces the whole Closure into scratch.
+    float coat = c.coatAlpha, spec = c.alpha, thin = c.thinAlpha;
+    asm("" : "+v"(coat), "+v"(spec), "+v"(thin));
     float alpha;
-    if (lobe == LOBE_COAT) alpha = c.coatAlpha;
-    else if (lobe == LOBE_SPECULAR) alpha = c.alpha;
-    else if (lobe == LOBE_TRANSMISSION) alpha = c.m.thinWalled != 0 ? c.thinAlpha : c.alpha;
+    if (lobe == LOBE_COAT) alpha = coat;
+    else if (lobe == LOBE_SPECULAR) alpha = spec;
+    else if (lobe == LOBE_TRANSMISSION) alpha = c.m.thinWalled != 0 ? thin : spec;
     else return 1.0f;
     return saturate(sqrtf(alpha) / 0.15f);
 }

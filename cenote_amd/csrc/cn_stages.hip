@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     v3 o, d;
     float cf;
     camera_ray(job.cam, job.width, job.height, pixel, sample, &o, &d, &cf);
-    p.origin[i] = F4(o, 0.0f);
+    p.origin[i] = F4(o, __uint_as_float(job.max_bounces));  // w: the path's depth cap
     p.direction[i] = F4(d, cf);
     p.throughput[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     p.key[i] = make_uint2(pixel, sample);
@@ -141,41 +141,68 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     }
 }
 
-// --- intersect (intersect.slang:45-88) ----------------------------------------------
+// --- trace: intersect (intersect.slang:45-88) fused with the previous
+// bounce's trace_shadow (trace_shadow.slang:27-72) ------------------------------
+//
+// Bounce b's extension rays and bounce b-1's shadow rays are independent, so
+// one launch walks both: items [0, n_ray) are rays, [n_ray, n_ray + n_shadow)
+// are shadow records. The reference's pass order is kept where it shows:
+// a slot's shadow add from bounce b-1 lands before bounce b's shading adds.
+// Two half-empty launches per bounce become one fuller one, which is what
+// the deep, sparse bounces need.
 
-__global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_intersect(SceneView s, PathPool p, uint32_t* counters, uint32_t base,
-                                                     uint32_t range_count, uint32_t bounce, int counting) {
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                           uint32_t range_count, uint32_t bounce, int counting) {
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
-    const uint32_t n = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
+    const uint32_t n_ray = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
+    const uint32_t n_shadow = bounce == 0 ? 0u : counters[(bounce - 1) * Q_COUNT + Q_SHADOW];
+    const uint32_t n = n_ray + n_shadow;
     const uint32_t* rays = p.ray_queue[bounce & 1];
     const uint32_t lanes = gridDim.x * BLOCK;
     Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
     const uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
-    uint32_t nodes = 0, tris = 0;
+    uint32_t ray_nodes = 0, ray_tris = 0, shadow_nodes = 0, shadow_tris = 0;
     for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
         const uint32_t i = first + threadIdx.x;
-        const bool valid = i < n;
+        const bool is_ray = i < n_ray;
         uint32_t path = 0;
         bool hit = false;
-        if (valid) {
+        if (is_ray) {
             path = bounce == 0 ? i : rays[i];
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
             const uint2 key = p.key[path];
-            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, nodes, tris);
+            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+        } else if (i < n) {
+            const uint32_t j = i - n_ray;
+            const float4 o4 = p.shadow_origin[j];
+            const float4 d4 = p.shadow_dir[j];
+            const float4 c4 = p.shadow_contrib[j];
+            const uint32_t target = __float_as_uint(c4.w);
+            // Any solid non-target hit short of the light's own distance
+            // (tmax / 1.001 for a triangle target) lies in front of the
+            // target, so it occludes without finding the closest hit.
+            const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
+            const float att = trace_visibility(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[j], t_early, st,
+                                               shadow_nodes, shadow_tris);
+            if (att >= 1e-4f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
         }
-        const uint32_t hs = wave_claim(&counters[bounce * Q_COUNT + Q_HIT], valid && hit);
-        const uint32_t ms = wave_claim(&counters[bounce * Q_COUNT + Q_MISS], valid && !hit);
-        if (valid) {
-            if (hit) p.hit_queue[hs] = path;
-            else p.miss_queue[ms] = path;
+        if (first < n_ray) {  // uniform per block: some wave of this block holds rays
+            const uint32_t hs = wave_claim(&counters[bounce * Q_COUNT + Q_HIT], is_ray && hit);
+            const uint32_t ms = wave_claim(&counters[bounce * Q_COUNT + Q_MISS], is_ray && !hit);
+            if (is_ray) {
+                if (hit) p.hit_queue[hs] = path;
+                else p.miss_queue[ms] = path;
+            }
         }
     }
     if (counting) {
-        wave_add_u64(&p.traversal_counts[0], nodes);
-        wave_add_u64(&p.traversal_counts[1], tris);
+        wave_add_u64(&p.traversal_counts[0], ray_nodes);
+        wave_add_u64(&p.traversal_counts[1], ray_tris);
+        wave_add_u64(&p.traversal_counts[2], shadow_nodes);
+        wave_add_u64(&p.traversal_counts[3], shadow_tris);
     }
 }
 
@@ -276,27 +303,30 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
     return recs[i];
 }
 
-// The BSDF tables are staged into LDS once per workgroup — 67,712 bytes
-// that every closure evaluation of the block reads ~20 times — so the
-// workgroup is wide (1024 lanes, 16 waves) to amortise the staging and keep
-// a CU's LDS for two such groups.
-constexpr uint32_t SHADE_BLOCK = 1024;
-
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
-                                                         uint32_t base, uint32_t bounce, uint32_t max_bounces,
-                                                         uint32_t light_sampling) {
-    __shared__ float lds_tables[T_FLOATS];
+// Launch-shape variants of shade_surface, picked per renderer (the
+// autotuner measures them). The BSDF tables — 67,712 bytes that each
+// closure evaluation reads ~20 times — are staged into LDS once per
+// workgroup, so wide groups amortise the staging and a CU's LDS holds two
+// groups; the last variant reads them through the caches instead and is not
+// limited by LDS.
+template <uint32_t SB, bool LDS_TABLES>
+__global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                      uint32_t base, uint32_t bounce, uint32_t light_sampling) {
     const uint32_t n = counters[bounce * Q_COUNT + Q_HIT];
-    if (blockIdx.x * SHADE_BLOCK >= n) return;  // uniform per block: nothing to shade here
-    for (uint32_t k = threadIdx.x; k < T_FLOATS / 4; k += SHADE_BLOCK) {
-        reinterpret_cast<float4*>(lds_tables)[k] = reinterpret_cast<const float4*>(s.tables)[k];
+    if (blockIdx.x * SB >= n) return;  // uniform per block: nothing to shade here
+    const float* tables = s.tables;
+    if constexpr (LDS_TABLES) {
+        __shared__ float lds_tables[T_FLOATS];
+        for (uint32_t k = threadIdx.x; k < T_FLOATS / 4; k += SB) {
+            reinterpret_cast<float4*>(lds_tables)[k] = reinterpret_cast<const float4*>(s.tables)[k];
+        }
+        __syncthreads();
+        tables = lds_tables;
     }
-    __syncthreads();
-    const float* tables = lds_tables;
     const bool aovOn = tg.albedo != nullptr;
     uint32_t* next_rays = p.ray_queue[(bounce + 1) & 1];
 
-    for (uint32_t first = blockIdx.x * SHADE_BLOCK; first < n; first += gridDim.x * SHADE_BLOCK) {
+    for (uint32_t first = blockIdx.x * SB; first < n; first += gridDim.x * SB) {
         const uint32_t i = first + threadIdx.x;
         const bool valid = i < n;
         bool push_ray = false, push_shadow = false;
@@ -311,7 +341,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
             const uint4 hit = p.hit[path];
             const float4 dir4 = p.direction[path];
             const v3 direction = xyz(dir4);
-            const v3 origin = xyz(p.origin[path]);
+            const float4 origin4 = p.origin[path];
+            const v3 origin = xyz(origin4);
+            const uint32_t depth_cap = __float_as_uint(origin4.w);
             const InstanceRecord inst = load_instance(s.instances, hit.x);
             const Mat material = load_material(s.materials, hit.x);
             const Spawn spawn = spawn_point(s, inst, hit.y, __uint_as_float(hit.z), __uint_as_float(hit.w), direction);
@@ -361,7 +393,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
             const v3 aovAlbedo = vclamp(closure.aovAlbedo, 0.0f, 1.0f);
             bool terminal = false;
 
-            if (bounce + 1 >= max_bounces) {
+            if (bounce + 1 >= depth_cap) {
                 terminal = true;
             } else {
                 // Next-event estimation (shade_surface.slang:363-446).
@@ -446,7 +478,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
                         medium = medium == hit.x ? MEDIUM_NONE : hit.x;
                     }
                     const bool transmitted = scatter.wi.z < 0.0f;
-                    p.origin[path] = F4(transmitted ? spawn.below : spawn.position, 0.0f);
+                    p.origin[path] = F4(transmitted ? spawn.below : spawn.position, origin4.w);
                     p.direction[path] = F4(wiWorld, 0.0f);
                     p.throughput[path] = F4(scattered, scatter.pdf);
                     p.state[path] = (scatter.lobe << 24) | (medium & 0xffffffu);
@@ -471,37 +503,6 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade_surface(SceneView s, Path
             p.shadow_contrib[ss] = F4(sh_contrib, __uint_as_float(sh_light));
             p.shadow_prim[ss] = sh_prim;
         }
-    }
-}
-
-// --- trace_shadow (trace_shadow.slang:27-72) ------------------------------------------
-
-__global__ __launch_bounds__(BLOCK) void k_trace_shadow(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
-                                                        uint32_t bounce, int counting) {
-    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
-    const uint32_t n = counters[bounce * Q_COUNT + Q_SHADOW];
-    const uint32_t lanes = gridDim.x * BLOCK;
-    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
-    uint32_t nodes = 0, tris = 0;
-    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
-        const uint32_t i = first + threadIdx.x;
-        if (i >= n) continue;
-        const float4 o4 = p.shadow_origin[i];
-        const float4 d4 = p.shadow_dir[i];
-        const float4 c4 = p.shadow_contrib[i];
-        const uint32_t target = __float_as_uint(c4.w);
-        // Any solid non-target hit short of the light's own distance
-        // (tmax / 1.001 for a triangle target) lies in front of the target,
-        // so it occludes without finding the closest hit.
-        const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
-        const float att = trace_visibility(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[i], t_early, st, nodes,
-                                           tris);
-        if (att < 1e-4f) continue;
-        add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
-    }
-    if (counting) {
-        wave_add_u64(&p.traversal_counts[2], nodes);
-        wave_add_u64(&p.traversal_counts[3], tris);
     }
 }
 
@@ -534,6 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restric
         const bool aov = tg.albedo != nullptr && job.albedo_sum != nullptr;
         float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nn = a;
         float d = 0.0f;
+        float m = (job.moment_sum && !reset) ? job.moment_sum[px] : 0.0f;
         if (aov && !reset) {
             a = job.albedo_sum[px];
             nn = job.normal_sum[px];
@@ -544,6 +546,10 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restric
             const bool first = reset && k == 0;
             const float4 cb = guarded(tg.radiance[slot]);
             b = first ? cb : add4(b, cb);
+            if (job.moment_sum) {
+                const float y = (cb.x * LUMA_R + cb.y * LUMA_G) + cb.z * LUMA_B;
+                m = first ? y * y : m + y * y;
+            }
             if (aov) {
                 const float4 ca = guarded(tg.albedo[slot]);
                 a = first ? ca : add4(a, ca);
@@ -555,6 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restric
             }
         }
         job.beauty_sum[px] = b;
+        if (job.moment_sum) job.moment_sum[px] = m;
         if (aov) {
             job.albedo_sum[px] = a;
             job.normal_sum[px] = nn;
@@ -571,6 +578,31 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(FilmView f, uint32_t pixels, 
         f.normal[i] = make_float4(nn.x / samples, nn.y / samples, nn.z / samples, nn.w / samples);
         f.depth[i] = f.depth_sum[i] / samples;
     }
+}
+
+// The estimated relative MSE of a film's running mean, for noise-target
+// stopping: per pixel, the variance of the mean luminance over its squared
+// mean plus 1e-2 (the relMSE denominator), from the luminance sum and the
+// squared-luminance sum; one f64 partial sum per block, summed on the host.
+__global__ __launch_bounds__(BLOCK) void k_noise(const float4* __restrict__ sums, const float* __restrict__ moments,
+                                                 uint32_t pixels, float samples, double* partials) {
+    __shared__ double lane_sums[BLOCK];
+    double acc = 0.0;
+    const double n = samples;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pixels; i += gridDim.x * BLOCK) {
+        const float4 b = sums[i];
+        const double mean = ((double(b.x) * LUMA_R + double(b.y) * LUMA_G) + double(b.z) * LUMA_B) / n;
+        const double second = double(moments[i]) / n;
+        const double var_mean = fmax(0.0, second - mean * mean) / (n - 1.0);
+        acc += var_mean / (mean * mean + 1e-2);
+    }
+    lane_sums[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) lane_sums[threadIdx.x] += lane_sums[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = lane_sums[0];
 }
 
 // --- diagnostics ----------------------------------------------------------------------
@@ -629,31 +661,44 @@ hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t max_bounces, uint32_t light_sampling, int counting,
+                         uint32_t bounce, uint32_t light_sampling, int counting,
                          Profiler* prof) {
     const uint32_t trace_grid = shape.trace_blocks;
-    uint32_t shade_grid = (range_count + SHADE_BLOCK - 1) / SHADE_BLOCK;
-    if (shade_grid > shape.shade_blocks) shade_grid = shape.shade_blocks;
-    if (shade_grid < 1) shade_grid = 1;
     const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
-    if (prof) prof->begin(K_INTERSECT);
-    hipLaunchKernelGGL(k_intersect, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, counters, base, range_count,
-                       bounce, counting);
-    if (prof) prof->end(K_INTERSECT);
+    // Bounce b's rays and bounce b-1's shadow rays share one launch; the
+    // last bounce shades terminally and leaves no shadow rays behind.
+    if (prof) prof->begin(K_TRACE);
+    hipLaunchKernelGGL(k_trace, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count, bounce,
+                       counting);
+    if (prof) prof->end(K_TRACE);
     if (prof) prof->begin(K_SHADE_MISS);
     hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
                        light_sampling);
     if (prof) prof->end(K_SHADE_MISS);
     if (prof) prof->begin(K_SHADE_SURFACE);
-    hipLaunchKernelGGL(k_shade_surface, dim3(shade_grid), dim3(SHADE_BLOCK), 0, stream, s, p, tg, counters, base,
-                       bounce, max_bounces, light_sampling);
-    if (prof) prof->end(K_SHADE_SURFACE);
-    if (bounce + 1 < max_bounces) {
-        if (prof) prof->begin(K_TRACE_SHADOW);
-        hipLaunchKernelGGL(k_trace_shadow, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, bounce,
-                           counting);
-        if (prof) prof->end(K_TRACE_SHADOW);
+    const uint32_t sb = shade_block(shape.shade_variant);
+    uint32_t shade_grid = (range_count + sb - 1) / sb;
+    if (shade_grid > shape.shade_blocks) shade_grid = shape.shade_blocks;
+    if (shade_grid < 1) shade_grid = 1;
+    switch (shape.shade_variant) {
+        case SHADE_1024_LDS:
+            hipLaunchKernelGGL((k_shade_surface<1024, true>), dim3(shade_grid), dim3(1024), 0, stream, s, p, tg, counters,
+                               base, bounce, light_sampling);
+            break;
+        case SHADE_512_LDS:
+            hipLaunchKernelGGL((k_shade_surface<512, true>), dim3(shade_grid), dim3(512), 0, stream, s, p, tg, counters,
+                               base, bounce, light_sampling);
+            break;
+        case SHADE_256_LDS:
+            hipLaunchKernelGGL((k_shade_surface<256, true>), dim3(shade_grid), dim3(256), 0, stream, s, p, tg, counters,
+                               base, bounce, light_sampling);
+            break;
+        default:
+            hipLaunchKernelGGL((k_shade_surface<256, false>), dim3(shade_grid), dim3(256), 0, stream, s, p, tg,
+                               counters, base, bounce, light_sampling);
+            break;
     }
+    if (prof) prof->end(K_SHADE_SURFACE);
     return hipGetLastError();
 }
 
@@ -668,6 +713,12 @@ hipError_t launch_accumulate(hipStream_t stream, const JobRecord* jobs, uint32_t
 
 hipError_t launch_resolve(hipStream_t stream, const FilmView& f, uint32_t pixels, float samples) {
     hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 4096)), dim3(BLOCK), 0, stream, f, pixels, samples);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise(hipStream_t stream, const float4* sums, const float* moments, uint32_t pixels, float samples,
+                        double* partials, uint32_t blocks) {
+    hipLaunchKernelGGL(k_noise, dim3(blocks), dim3(BLOCK), 0, stream, sums, moments, pixels, samples, partials);
     return hipGetLastError();
 }
 

@@ -69,10 +69,13 @@ class Context:
 class Scene:
     """A scene resident on the device (scene/mod.rs:161-219)."""
 
-    def __init__(self, ctx: Context, data: SceneData):
+    def __init__(self, ctx: Context, data: SceneData, bvh_builder: int = _abi.BVH_DEVICE):
+        """``bvh_builder``: ``_abi.BVH_DEVICE`` (default) or ``_abi.BVH_HOST``;
+        both build the same tree."""
         self.ctx = ctx
         self._lib = ctx._lib
         desc, keep = data.desc()
+        desc.bvh_builder = bvh_builder
         handle = C.c_void_p()
         check(self._lib.cn_scene_create(ctx.handle, C.byref(desc), C.byref(handle)))
         del keep
@@ -81,6 +84,17 @@ class Scene:
     def set_camera(self, camera: CameraData):
         cam = camera.record()
         check(self._lib.cn_scene_set_camera(self.handle, C.byref(cam)))
+
+    def bvh(self):
+        """The resident BVH: nodes as (N, 32) u32 words (128-byte records)
+        and leaf-order triangles as (M, 12) u32 words (48-byte records)."""
+        nn, nt = C.c_uint32(), C.c_uint32()
+        check(self._lib.cn_scene_bvh(self.handle, None, 0, C.byref(nn), None, 0, C.byref(nt)))
+        nodes = np.zeros((nn.value, 32), np.uint32)
+        tris = np.zeros((nt.value, 12), np.uint32)
+        check(self._lib.cn_scene_bvh(self.handle, nodes.ctypes.data, nn.value, C.byref(nn), tris.ctypes.data,
+                                     nt.value, C.byref(nt)))
+        return nodes, tris
 
     def info(self) -> _abi.SceneInfo:
         info = _abi.SceneInfo()
@@ -210,14 +224,18 @@ class Renderer:
 
     def render_batch(self, views):
         """Many views in shared waves: ``views`` is a sequence of
-        ``(scene, film, camera, samples)``; each film gets ``samples`` more
-        samples of its view, bitwise as sequential ``accumulate`` calls."""
+        ``(scene, film, camera, samples)`` or ``(scene, film, camera,
+        samples, max_bounces)``; each film gets ``samples`` more samples of
+        its view, bitwise as sequential ``accumulate`` calls on a renderer of
+        that depth."""
         jobs = (_abi.ViewJob * max(1, len(views)))()
-        for j, (scene, film, camera, samples) in zip(jobs, views):
+        for j, view in zip(jobs, views):
+            scene, film, camera, samples = view[:4]
             j.scene = scene.handle.value
             j.film = film.handle.value
             j.camera = camera.record()
             j.samples = samples
+            j.max_bounces = view[4] if len(view) > 4 else 0
         check(self._lib.cn_render_batch(self.handle, jobs, len(views)))
 
     def accumulate_sample(self, film: Film):
@@ -238,6 +256,27 @@ class Renderer:
     def samples_per_wave(self, width: int, height: int) -> int:
         """How many samples of a width x height view one wave carries."""
         return max(1, self.capacity // (width * height))
+
+    @property
+    def shape(self) -> dict:
+        """The launch shape: trace_blocks, shade_variant, shade_blocks, light_blocks."""
+        sh = _abi.LaunchShape()
+        check(self._lib.cn_renderer_get_shape(self.handle, C.byref(sh)))
+        return {name: getattr(sh, name) for name, _ in _abi.LaunchShape._fields_}
+
+    @shape.setter
+    def shape(self, value: dict):
+        sh = _abi.LaunchShape(**{**self.shape, **value})
+        check(self._lib.cn_renderer_set_shape(self.handle, C.byref(sh)))
+
+    def set_graphs(self, enabled: bool):
+        """Replay repeated wave shapes from captured hipGraphs."""
+        check(self._lib.cn_renderer_set_graphs(self.handle, 1 if enabled else 0))
+
+    def graph_stats(self):
+        captures, replays = C.c_uint64(), C.c_uint64()
+        check(self._lib.cn_renderer_graph_stats(self.handle, C.byref(captures), C.byref(replays)))
+        return captures.value, replays.value
 
     def set_counters(self, enabled: bool):
         check(self._lib.cn_renderer_set_counters(self.handle, 1 if enabled else 0))

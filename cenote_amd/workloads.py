@@ -14,6 +14,9 @@
   absorption, coat over conductor, and fuzz — plus 10,000 emissive triangles
   of log-uniform radiance in [1e-2, 1e2] and the chart's sky.
 * ``furnace()`` — render/tests.rs:38-78: one big plane under a 0.5 sky.
+* ``batched_views()`` — 64 seeded views of the chart and heavy scenes:
+  pixel counts uniform in [128^2, 1024^2], aspect uniform in [1, 16/9], spp
+  uniform in 1..64, cameras turned about each scene's look-at point.
 """
 
 from __future__ import annotations
@@ -234,3 +237,30 @@ def furnace(material: Material, center=(0.0, 0.0, 0.0), scale: float = 1.0, aper
 
 def build(name: str, seed: int = 0) -> SceneData:
     return {"cornell": cornell, "chart": lambda: chart(seed), "heavy": lambda: heavy(seed)}[name]()
+
+
+def batched_views(cameras: dict, seed: int = 0, count: int = 64, scenes=("chart", "heavy")) -> list:
+    """Seeded batched-view specs: (scene name, width, height, spp, depth,
+    camera), alternating scenes."""
+    import dataclasses
+    import math
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        name = scenes[k % len(scenes)]
+        pixels = rng.uniform(128.0 * 128.0, 1024.0 * 1024.0)
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        width = max(1, int(round(math.sqrt(pixels * aspect))))
+        height = max(1, int(round(pixels / width)))
+        spp = int(rng.integers(1, 65))
+        cam = cameras[name]
+        angle = rng.uniform(-0.3, 0.3)
+        p = np.asarray(cam.position, np.float64)
+        c = np.asarray(cam.look_at, np.float64)
+        v = p - c
+        x = v[0] * math.cos(angle) + v[2] * math.sin(angle)
+        z = -v[0] * math.sin(angle) + v[2] * math.cos(angle)
+        turned = dataclasses.replace(cam, position=(float(c[0] + x), float(p[1]), float(c[2] + z)))
+        out.append((name, width, height, spp, CONFIGS[name]["max_bounces"], turned))
+    return out

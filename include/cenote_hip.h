@@ -172,7 +172,15 @@ typedef struct cn_scene_desc {
     uint32_t delta_light_count;
     cn_environment environment;
     cn_camera camera;
+    uint32_t bvh_builder;  /* CN_BVH_DEVICE (0) or CN_BVH_HOST: both grow the same tree */
+    uint32_t pad;
 } cn_scene_desc;
+
+/* Who builds the scene's BVH (no reference counterpart: the reference's
+ * driver builds its acceleration structures, gpu/accel.rs:167-269). The
+ * device builder is the default; the host builder is the same algorithm on
+ * the CPU, kept as a cross-check. */
+enum { CN_BVH_DEVICE = 0, CN_BVH_HOST = 1 };
 
 /* The 80-byte light record as the kernels read it (lights.rs:81-115). */
 typedef struct cn_light_record {
@@ -203,7 +211,7 @@ typedef struct cn_scene_info {
     uint32_t bvh_node_bytes;
     uint32_t bvh_triangle_bytes;
     uint32_t bvh_max_depth;
-    uint32_t pad;
+    uint32_t bvh_builder;      /* the CN_BVH_* that built it */
     double bvh_build_seconds;
     double prep_seconds;
 } cn_scene_info;
@@ -259,6 +267,10 @@ int cn_scene_destroy(cn_scene* scene);
 /* Camera edits between frames (Scene::camera_mut, scene/mod.rs:391-394). */
 int cn_scene_set_camera(cn_scene* scene, const cn_camera* camera);
 int cn_scene_info_get(cn_scene* scene, cn_scene_info* info);
+/* The resident BVH as raw records: 128-byte nodes and 48-byte leaf-order
+ * triangles (either output may be NULL; *count gets the record count). */
+int cn_scene_bvh(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t* node_count, void* triangles,
+                 uint32_t triangle_capacity, uint32_t* triangle_count);
 /* Host copies of the bit-exact tables, for parity checks. */
 int cn_scene_light_records(cn_scene* scene, cn_light_record* out, uint32_t capacity,
                            uint32_t* count);
@@ -276,6 +288,33 @@ int cn_renderer_destroy(cn_renderer* renderer);
  * nothing. The reference allocates its whole pool up front
  * (wavefront.rs:486-540); here the pool otherwise grows as waves need it. */
 int cn_renderer_reserve(cn_renderer* renderer, uint32_t paths, int aov);
+/* Launch shape of the stage kernels (no reference counterpart: the
+ * reference's dispatch sizes come from its queue headers). The autotuner
+ * measures candidates and sets the fastest; every shape renders bitwise the
+ * same images. */
+enum {
+    CN_SHADE_1024_LDS = 0,   /* 1024-lane groups, BSDF tables staged in LDS */
+    CN_SHADE_512_LDS = 1,
+    CN_SHADE_256_LDS = 2,
+    CN_SHADE_256_GLOBAL = 3  /* 256-lane groups, tables read through the caches */
+};
+typedef struct cn_launch_shape {
+    uint32_t trace_blocks;   /* 256-lane workgroups of the persistent trace kernel, 1..65536 */
+    uint32_t shade_variant;  /* a CN_SHADE_* value */
+    uint32_t shade_blocks;   /* most workgroups of shade_surface, 1..65536 */
+    uint32_t light_blocks;   /* most 256-lane workgroups of shade_miss, 1..65536 */
+} cn_launch_shape;
+int cn_renderer_get_shape(cn_renderer* renderer, cn_launch_shape* out);
+int cn_renderer_set_shape(cn_renderer* renderer, const cn_launch_shape* shape);
+/* Replay each wave's launch sequence from a captured hipGraph when a wave
+ * of the same shape (views, sizes, sample counts per job, scene groups)
+ * was captured before; progressive frames of the same views then cost one
+ * graph launch each. Off while profiling or counting. No reference
+ * counterpart (the reference records a fresh command buffer per wave,
+ * wavefront.rs:607-629). */
+int cn_renderer_set_graphs(cn_renderer* renderer, int enabled);
+/* Waves captured into new graphs and waves replayed from the cache. */
+int cn_renderer_graph_stats(cn_renderer* renderer, uint64_t* captures, uint64_t* replays);
 /* Count BVH node and triangle fetches in later waves (slower). */
 int cn_renderer_set_counters(cn_renderer* renderer, int enabled);
 int cn_renderer_wave_counts(cn_renderer* renderer, cn_wave_counts* out);
@@ -314,13 +353,14 @@ int cn_accumulate(cn_renderer* renderer, cn_scene* scene, cn_film* film);
 int cn_accumulate_async(cn_renderer* renderer, cn_scene* scene, cn_film* film,
                         uint32_t count);
 /* One view of a batch: `samples` more samples of `camera` on `scene`,
- * folded into `film` (whose size sets the view's resolution). */
+ * folded into `film` (whose size sets the view's resolution), with paths
+ * capped at `max_bounces` (0 = the renderer's; at most the renderer's). */
 typedef struct cn_view_job {
     cn_scene* scene;
     cn_film* film;
     cn_camera camera;
     uint32_t samples;
-    uint32_t pad;
+    uint32_t max_bounces;
 } cn_view_job;
 /* Many views in shared waves (no reference counterpart: the reference
  * renders one view per Renderer::accumulate, render/mod.rs:245-259).
@@ -329,6 +369,77 @@ typedef struct cn_view_job {
  * cn_accumulate calls on its view. Films must be distinct. Blocks until
  * done. */
 int cn_render_batch(cn_renderer* renderer, const cn_view_job* views, uint32_t count);
+/* --- Serving (no reference counterpart; the nearest is the progressive
+ * Session, render/session.rs:480-489, which renders one view) -----------
+ *
+ * A request loop that admits progressive render requests while their films
+ * fit an HBM budget, keeps scenes resident, and interleaves the admitted
+ * requests into shared waves at sample granularity: each step builds one
+ * wave in which every request without a sample gets its first one, every
+ * other unfinished request one more, and leftover pool room more samples in
+ * admission order. Two waves stay in flight. A request's film receives its
+ * samples in index order, so its result is bitwise the same alone as in any
+ * mix. Times are seconds on the server's steady clock. */
+typedef struct cn_server cn_server;
+
+enum {
+    CN_REQUEST_QUEUED = 0,   /* waiting for budget */
+    CN_REQUEST_ACTIVE = 1,   /* admitted, samples in progress */
+    CN_REQUEST_DONE = 2,     /* target reached, result ready */
+    CN_REQUEST_RELEASED = 3  /* result taken, film returned to the server */
+};
+
+typedef struct cn_request_desc {
+    cn_scene* scene;
+    cn_camera camera;
+    uint32_t width, height;
+    uint32_t spp;            /* target samples (with rel_mse: the most samples) */
+    uint32_t max_bounces;    /* 0 = the renderer's; at most the renderer's */
+    float rel_mse;           /* > 0: stop once the estimated relMSE of the mean
+                              * luminance (variance of the mean over mean^2 + 1e-2,
+                              * averaged over pixels) is at most this; 0 = spp only */
+    float pad;
+    double arrival_seconds;  /* the request's arrival on the server clock; < 0 = now */
+} cn_request_desc;
+
+typedef struct cn_request_status {
+    uint64_t id;
+    uint32_t state;               /* a CN_REQUEST_* value */
+    uint32_t samples;             /* samples whose waves have completed */
+    double arrival_seconds;
+    double admitted_seconds;      /* -1 until admitted */
+    double first_frame_seconds;   /* -1 until the first sample's wave completed */
+    double done_seconds;          /* -1 until done */
+    double rel_mse_estimate;      /* -1 without a noise target or before 2 samples */
+} cn_request_status;
+
+typedef struct cn_server_stats {
+    uint64_t submitted, completed, waves, paths;
+    uint32_t queued, active;
+    uint64_t hbm_used_bytes;      /* films held by requests */
+} cn_server_stats;
+
+/* wave_paths: most paths per wave (0 = the renderer's capacity). */
+int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t wave_paths,
+                     cn_server** out);
+int cn_server_destroy(cn_server* server);
+int cn_server_clock(cn_server* server, double* seconds);
+int cn_server_submit(cn_server* server, const cn_request_desc* request, uint64_t* id);
+/* One scheduling step: admit, enqueue one wave, retire the oldest wave when
+ * two are in flight (or when there was nothing to enqueue). `busy` = queued
+ * + active requests + waves in flight (0: idle). */
+int cn_server_step(cn_server* server, uint32_t* busy);
+/* Step until idle. Fails with CN_E_ALLOC, instead of waiting forever, when
+ * queued requests cannot be admitted because finished requests' films
+ * (results not yet taken) hold the budget. */
+int cn_server_drain(cn_server* server);
+int cn_server_status(cn_server* server, uint64_t id, cn_request_status* out);
+int cn_server_stats_get(cn_server* server, cn_server_stats* out);
+/* The averages (sums / samples) of a done request; releases its film. Any
+ * output may be NULL. */
+int cn_server_result(cn_server* server, uint64_t id, float* beauty, float* albedo, float* normal,
+                     float* depth);
+
 /* Block until the renderer's stream has drained. */
 int cn_renderer_synchronize(cn_renderer* renderer);
 /* Film::averages (render/film.rs:183-197): sums / samples on the host.

@@ -124,3 +124,24 @@ def test_batch_rejects_a_film_twice_in_one_wave(ctx, scenes):
         r.render_batch([(scene, film, data.camera, 1), (scene, film, data.camera, 1)])
     film.close()
     r.close()
+
+
+def test_batch_mixes_views_of_different_depths(ctx, scenes):
+    """Each view keeps its own depth cap (its paths carry it), so one wave
+    can hold a depth-12 view and a depth-5 view."""
+    chart, hs = scenes["chart"]
+    cornell, cs = scenes["cornell"]
+    want_deep = sequential_sums(ctx, hs, chart.camera, 96, 54, 3, 12)
+    want_shallow = sequential_sums(ctx, cs, cornell.camera, 64, 48, 4, 5)
+    r = cn.Renderer(ctx, max_bounces=12)
+    deep, shallow = cn.Film(ctx, 96, 54), cn.Film(ctx, 64, 48)
+    r.render_batch([(hs, deep, chart.camera, 3, 12), (cs, shallow, cornell.camera, 4, 5)])
+    assert_same_sums(deep.sums(), want_deep)
+    assert_same_sums(shallow.sums(), want_shallow)
+    with pytest.raises(cn.CenoteError):
+        r.render_batch([(cs, shallow, cornell.camera, 1, 13)])
+    deep.close()
+    shallow.close()
+    r.close()
+    cs.set_camera(cornell.camera)
+    hs.set_camera(chart.camera)
