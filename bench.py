@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--workload", default="chart", choices=["cornell", "chart", "heavy", "batched", "serving"])
     ap.add_argument("--rate", type=float, default=10.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--requests", type=int, default=48, help="serving: requests in the stream")
+    ap.add_argument("--serving-requests", type=int, default=32,
+                    help="chart: also serve this many Poisson requests, served vs plain (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
@@ -389,6 +391,11 @@ def main():
     renderer.close()
     scene.close()
     ctx.close()
+    if args.workload == "chart" and args.serving_requests > 0:
+        # The metric's other half: the same GPU serving a Poisson request
+        # stream, continuous batching vs the plain one-frame-at-a-time loop.
+        served = serving_bench(argparse.Namespace(rate=args.rate, requests=args.serving_requests))
+        result["served_vs_plain"] = {k: served[k] for k in ("config", "served", "plain", "speedup_vs_plain")}
     print(json.dumps(result), flush=True)
 
 
