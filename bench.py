@@ -263,6 +263,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=32,
                     help="chart: also serve this many Poisson requests, served vs plain (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--capacity", type=int, default=0, help="path-pool capacity (0: the library's default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -297,7 +298,7 @@ def main():
     info = scene.info()
     log(f"device {name} ({cus} CUs); scene prep {info.prep_seconds:.2f} s (BVH {info.bvh_build_seconds:.2f} s, "
         f"{info.bvh_node_count} nodes, depth {info.bvh_max_depth})")
-    renderer = cn.Renderer(ctx, max_bounces=depth)
+    renderer = cn.Renderer(ctx, capacity=args.capacity, max_bounces=depth)
     film = cn.Film(ctx, width, height)
     pixels = width * height
     steps = args.steps
@@ -367,7 +368,7 @@ def main():
                         + (", AOVs" if aov else ""),
             "triangles": data.triangle_count,
             "pixels": pixels,
-            "path_pool": cn.POOL_CAPACITY,
+            "path_pool": renderer.capacity,
             "form": "plain (one progressive frame, one sample per step; waves of several samples)",
         },
         "grays_per_s": round(grays, 4),

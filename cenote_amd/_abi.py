@@ -184,6 +184,9 @@ SHADE_1024_LDS = 0
 SHADE_512_LDS = 1
 SHADE_256_LDS = 2
 SHADE_256_GLOBAL = 3
+SHADE_512_LDS_SORTED = 4
+SHADE_256_LDS_SORTED = 5
+SHADE_256_GLOBAL_SORTED = 6
 
 
 class LaunchShape(C.Structure):
@@ -310,6 +313,9 @@ SIGNATURES = {
     "cn_server_status": [_P, C.c_uint64, C.POINTER(RequestStatus)],
     "cn_server_stats_get": [_P, C.POINTER(ServerStats)],
     "cn_server_result": [_P, C.c_uint64, _FP, _FP, _FP, _FP],
+    "cn_write_exr": [C.c_char_p, C.c_uint32, C.c_uint32, _FP],
+    "cn_write_aov_exr": [C.c_char_p, C.c_uint32, C.c_uint32, _FP, _FP, _FP, _FP],
+    "cn_read_exr": [C.c_char_p, _UP, _UP, _FP, C.c_size_t],
     "cn_renderer_synchronize": [_P],
     "cn_film_averages": [_P, _FP, _FP, _FP, _FP],
     "cn_film_sums": [_P, _FP, _FP, _FP, _FP],

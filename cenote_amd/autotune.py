@@ -24,6 +24,9 @@ SHADE_VARIANTS = {
     _abi.SHADE_512_LDS: 512,
     _abi.SHADE_256_LDS: 256,
     _abi.SHADE_256_GLOBAL: 256,
+    _abi.SHADE_512_LDS_SORTED: 512,
+    _abi.SHADE_256_LDS_SORTED: 256,
+    _abi.SHADE_256_GLOBAL_SORTED: 256,
 }
 
 

@@ -1,0 +1,83 @@
+"""``python -m cenote_amd.cli render [scene] --spp --width --height --depth
+--out`` — the headless render of cenote-cli (crates/cenote-cli/src/main.rs:
+30-71, 103-244): accumulate a scene to ``--spp`` samples and write the
+linear averages as one multi-layer EXR (beauty, albedo and normal guides,
+first-hit depth; output.rs:84-136).
+
+Scenes are the seeded built-ins (``cornell``, ``chart`` — the default, the
+material chart standing in for the reference's built-in demo — and
+``heavy``); their settings fill any flag left out, as the reference's scene
+settings do (main.rs:109-132). Scene files (``.ron``/``.pbrt``) are not read
+by this build.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+
+def _positive(lo, hi=None):
+    def parse(text):
+        v = int(text)
+        if v < lo or (hi is not None and v > hi):
+            raise argparse.ArgumentTypeError(f"must be in {lo}..{hi if hi is not None else ''}")
+        return v
+    return parse
+
+
+def build_parser() -> argparse.ArgumentParser:
+    ap = argparse.ArgumentParser(prog="cenote", description="Cenote: a GPU path tracer (MI355X)")
+    sub = ap.add_subparsers(dest="command", required=True)
+    r = sub.add_parser("render", help="render a scene to an EXR (the built-in chart when none is named)")
+    r.add_argument("scene", nargs="?", default=None, help="cornell, chart or heavy (default: chart)")
+    r.add_argument("--spp", type=_positive(1), help="samples per pixel (default: the scene's)")
+    r.add_argument("--width", type=_positive(1), help="output width in pixels (default: the scene's)")
+    r.add_argument("--height", type=_positive(1), help="output height in pixels (default: the scene's)")
+    r.add_argument("--depth", type=_positive(1, 255), help="maximum path length in bounces (default: the scene's)")
+    r.add_argument("--out", default="render.exr", help="output EXR path")
+    return ap
+
+
+def render(args) -> int:
+    import cenote_amd as cn
+    from cenote_amd import output, workloads
+
+    name = args.scene or "chart"
+    if name not in workloads.CONFIGS:
+        print(f"error: unknown scene {name!r} (built-ins: {', '.join(sorted(workloads.CONFIGS))}; "
+              "scene files are not read by this build)", file=sys.stderr)
+        return 2
+    cfg = workloads.CONFIGS[name]
+    width = args.width or cfg["width"]
+    height = args.height or cfg["height"]
+    spp = args.spp or cfg["spp"]
+    depth = args.depth or cfg["max_bounces"]
+    ctx = cn.Context(0)
+    scene = cn.Scene(ctx, workloads.build(name))
+    renderer = cn.Renderer(ctx, max_bounces=depth)
+    film = cn.Film(ctx, width, height)
+    t0 = time.perf_counter()
+    renderer.accumulate(scene, film, count=spp)
+    beauty, albedo, normal, z = film.averages()
+    seconds = time.perf_counter() - t0
+    output.write_aov_exr(args.out, beauty, albedo, normal, z)
+    print(f"wrote {args.out} ({width}×{height}, {spp} spp; layers: beauty, albedo, normal, Z) "
+          f"in {seconds:.2f} s")
+    film.close()
+    renderer.close()
+    scene.close()
+    ctx.close()
+    return 0
+
+
+def main(argv=None) -> int:
+    args = build_parser().parse_args(argv)
+    if args.command == "render":
+        return render(args)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main())

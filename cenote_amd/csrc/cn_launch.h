@@ -16,11 +16,13 @@ constexpr uint32_t STACK_MAX = 128;
 constexpr uint32_t STACK_SPILL = STACK_MAX - STACK_LDS;
 
 // shade_surface launch variants: workgroup size and where the BSDF tables live.
+// The _SORTED variants order each workgroup's hits by material class first.
 enum ShadeVariant : uint32_t { SHADE_1024_LDS = 0, SHADE_512_LDS = 1, SHADE_256_LDS = 2, SHADE_256_GLOBAL = 3,
-                               SHADE_VARIANTS = 4 };
+                               SHADE_512_LDS_SORTED = 4, SHADE_256_LDS_SORTED = 5, SHADE_256_GLOBAL_SORTED = 6,
+                               SHADE_VARIANTS = 7 };
 
 inline uint32_t shade_block(uint32_t variant) {
-    return variant == SHADE_1024_LDS ? 1024u : variant == SHADE_512_LDS ? 512u : 256u;
+    return variant == SHADE_1024_LDS ? 1024u : (variant == SHADE_512_LDS || variant == SHADE_512_LDS_SORTED) ? 512u : 256u;
 }
 
 // Grid sizes: the traversal kernels run a fixed, chip-filling grid and

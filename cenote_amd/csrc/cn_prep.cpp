@@ -59,6 +59,11 @@ cn_light_record fresh_record(uint32_t kind, v3 anchor, v3 e1, v3 e2, v3 emission
 
 float luminance(v3 c) { return c.x * 0.2722287f + c.y * 0.6740818f + c.z * 0.0536895f; }
 
+uint32_t material_class(const cn_material& m) {
+    return (m.metalness > 0.0f ? 1u : 0u) | (m.metalness < 1.0f ? 2u : 0u) | (m.transmission_weight > 0.0f ? 4u : 0u) |
+           (m.coat_weight > 0.0f ? 8u : 0u) | (m.fuzz_weight > 0.0f ? 16u : 0u);
+}
+
 std::vector<cn_light_record> build_light_table(const std::vector<EmissiveTriangle>& tris,
                                                const cn_delta_light* deltas, uint32_t delta_count,
                                                double* total_power) {

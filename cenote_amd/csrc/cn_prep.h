@@ -24,6 +24,11 @@ struct EmissiveTriangle {
 // color.rs:29-31
 float luminance(v3 c);
 
+// A material's lobe set as a class in 0..31 (metallic, dielectric base,
+// transmission, coat, fuzz): materials of one class take the same branches
+// through the closure.
+uint32_t material_class(const cn_material& m);
+
 // lights.rs:175-224, :229-231. Returns the records; *total_power is the
 // list's total selection weight.
 std::vector<cn_light_record> build_light_table(const std::vector<EmissiveTriangle>& tris,

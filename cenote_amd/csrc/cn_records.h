@@ -85,7 +85,7 @@ struct alignas(16) InstanceRecord {
     uint32_t vertex_base;
     uint32_t index_base;  // in triangles
     uint32_t flags;       // bits 0..7 = visibility mask, bit 8 = opaque
-    uint32_t pad[4];
+    uint32_t pad[4];      // pad[0]: the material class (material_class), for material-sorted shading
 };
 static_assert(sizeof(InstanceRecord) == 128, "two dwordx4 lines per instance");
 

@@ -190,6 +190,7 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
         r.vertex_base = vbase[inst.mesh];
         r.index_base = ibase[inst.mesh];
         r.flags = mask | (opaque ? 0x100u : 0u);
+        r.pad[0] = material_class(inst.material);
         r.light = LIGHT_NONE;
         materials[i] = inst.material;
         const v3 emission = V3(inst.material.emission[0], inst.material.emission[1], inst.material.emission[2]);

@@ -309,7 +309,15 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
 // workgroup, so wide groups amortise the staging and a CU's LDS holds two
 // groups; the last variant reads them through the caches instead and is not
 // limited by LDS.
-template <uint32_t SB, bool LDS_TABLES>
+// Material sorting (the SORT variants): each workgroup's hits are ordered by
+// their instance's material class (which lobes the closure has) with an LDS
+// counting sort before shading, so a wave64 shades one lobe set where it
+// can instead of every branch of the closure. The order in which a
+// workgroup shades its hits reaches no result: every add lands in the
+// path's own slot.
+constexpr uint32_t MATERIAL_CLASSES = 32;
+
+template <uint32_t SB, bool LDS_TABLES, bool SORT>
 __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
     const uint32_t n = counters[bounce * Q_COUNT + Q_HIT];
@@ -328,14 +336,38 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
 
     for (uint32_t first = blockIdx.x * SB; first < n; first += gridDim.x * SB) {
         const uint32_t i = first + threadIdx.x;
-        const bool valid = i < n;
+        bool valid = i < n;
+        uint32_t path = valid ? p.hit_queue[i] : 0u;
+        if constexpr (SORT) {
+            // Bucket MATERIAL_CLASSES holds the lanes past the queue's end.
+            __shared__ uint32_t bucket[MATERIAL_CLASSES + 1];
+            __shared__ uint32_t sorted[SB];
+            const uint32_t cls = valid ? s.instances[p.hit[path].x].pad[0] : MATERIAL_CLASSES;
+            if (threadIdx.x <= MATERIAL_CLASSES) bucket[threadIdx.x] = 0;
+            __syncthreads();
+            const uint32_t rank = atomicAdd(&bucket[cls], 1u);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t run = 0;
+                for (uint32_t c = 0; c <= MATERIAL_CLASSES; ++c) {
+                    const uint32_t k = bucket[c];
+                    bucket[c] = run;
+                    run += k;
+                }
+            }
+            __syncthreads();
+            sorted[bucket[cls] + rank] = valid ? path : 0xffffffffu;
+            __syncthreads();
+            path = sorted[threadIdx.x];
+            valid = path != 0xffffffffu;
+            if (!valid) path = 0;
+        }
         bool push_ray = false, push_shadow = false;
-        uint32_t path = 0, pixel = 0;
+        uint32_t pixel = 0;
         v3 sh_origin = S3(0.0f), sh_dir = S3(0.0f), sh_contrib = S3(0.0f);
         float sh_tmax = 0.0f;
         uint32_t sh_light = LIGHT_NONE, sh_prim = 0;
         if (valid) {
-            path = p.hit_queue[i];
             pixel = base + path;  // the path's target slot
             const uint2 key = p.key[path];
             const uint4 hit = p.hit[path];
@@ -680,24 +712,19 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     uint32_t shade_grid = (range_count + sb - 1) / sb;
     if (shade_grid > shape.shade_blocks) shade_grid = shape.shade_blocks;
     if (shade_grid < 1) shade_grid = 1;
+#define CN_SHADE_LAUNCH(SB, LDS, SORT)                                                                      \
+    hipLaunchKernelGGL((k_shade_surface<SB, LDS, SORT>), dim3(shade_grid), dim3(SB), 0, stream, s, p, tg, counters, \
+                       base, bounce, light_sampling)
     switch (shape.shade_variant) {
-        case SHADE_1024_LDS:
-            hipLaunchKernelGGL((k_shade_surface<1024, true>), dim3(shade_grid), dim3(1024), 0, stream, s, p, tg, counters,
-                               base, bounce, light_sampling);
-            break;
-        case SHADE_512_LDS:
-            hipLaunchKernelGGL((k_shade_surface<512, true>), dim3(shade_grid), dim3(512), 0, stream, s, p, tg, counters,
-                               base, bounce, light_sampling);
-            break;
-        case SHADE_256_LDS:
-            hipLaunchKernelGGL((k_shade_surface<256, true>), dim3(shade_grid), dim3(256), 0, stream, s, p, tg, counters,
-                               base, bounce, light_sampling);
-            break;
-        default:
-            hipLaunchKernelGGL((k_shade_surface<256, false>), dim3(shade_grid), dim3(256), 0, stream, s, p, tg,
-                               counters, base, bounce, light_sampling);
-            break;
+        case SHADE_1024_LDS: CN_SHADE_LAUNCH(1024, true, false); break;
+        case SHADE_512_LDS: CN_SHADE_LAUNCH(512, true, false); break;
+        case SHADE_256_LDS: CN_SHADE_LAUNCH(256, true, false); break;
+        case SHADE_256_GLOBAL: CN_SHADE_LAUNCH(256, false, false); break;
+        case SHADE_512_LDS_SORTED: CN_SHADE_LAUNCH(512, true, true); break;
+        case SHADE_256_LDS_SORTED: CN_SHADE_LAUNCH(256, true, true); break;
+        default: CN_SHADE_LAUNCH(256, false, true); break;
     }
+#undef CN_SHADE_LAUNCH
     if (prof) prof->end(K_SHADE_SURFACE);
     return hipGetLastError();
 }

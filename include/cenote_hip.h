@@ -296,7 +296,11 @@ enum {
     CN_SHADE_1024_LDS = 0,   /* 1024-lane groups, BSDF tables staged in LDS */
     CN_SHADE_512_LDS = 1,
     CN_SHADE_256_LDS = 2,
-    CN_SHADE_256_GLOBAL = 3  /* 256-lane groups, tables read through the caches */
+    CN_SHADE_256_GLOBAL = 3, /* 256-lane groups, tables read through the caches */
+    /* As above, each group first ordering its hits by material class: */
+    CN_SHADE_512_LDS_SORTED = 4,
+    CN_SHADE_256_LDS_SORTED = 5,
+    CN_SHADE_256_GLOBAL_SORTED = 6
 };
 typedef struct cn_launch_shape {
     uint32_t trace_blocks;   /* 256-lane workgroups of the persistent trace kernel, 1..65536 */
@@ -439,6 +443,21 @@ int cn_server_stats_get(cn_server* server, cn_server_stats* out);
  * output may be NULL. */
 int cn_server_result(cn_server* server, uint64_t id, float* beauty, float* albedo, float* normal,
                      float* depth);
+
+/* --- EXR output and input (output.rs:37-193) --------------------------
+ * Linear ACEScg files with the AP1 chromaticities attribute; single-part
+ * scanline, ZIP-compressed. cn_write_exr: row-major RGBA f32 (pixel (0,0)
+ * top-left) as f32 R/G/B/A (write_exr, output.rs:37-57). cn_write_aov_exr:
+ * the film's averages as one multi-layer file — f16 R/G/B/A, f32 Z, f16
+ * albedo.R/G/B and normal.X/Y/Z (write_aov_exr, output.rs:84-136).
+ * cn_read_exr: the first layer's R/G/B/A as f32, missing alpha = 1
+ * (read_exr, output.rs:148-193); scanline or single-level tiled files,
+ * NONE/RLE/ZIPS/ZIP, HALF/FLOAT/UINT channels. With rgba NULL it only
+ * reports the size; capacity is in floats. */
+int cn_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgba);
+int cn_write_aov_exr(const char* path, uint32_t width, uint32_t height, const float* beauty,
+                     const float* albedo, const float* normal, const float* depth);
+int cn_read_exr(const char* path, uint32_t* width, uint32_t* height, float* rgba, size_t capacity);
 
 /* Block until the renderer's stream has drained. */
 int cn_renderer_synchronize(cn_renderer* renderer);

@@ -33,7 +33,8 @@ def render_sums(ctx, scene, shape, width=128, height=72, samples=3):
 
 
 @pytest.mark.parametrize("variant", [_abi.SHADE_1024_LDS, _abi.SHADE_512_LDS, _abi.SHADE_256_LDS,
-                                     _abi.SHADE_256_GLOBAL])
+                                     _abi.SHADE_256_GLOBAL, _abi.SHADE_512_LDS_SORTED,
+                                     _abi.SHADE_256_LDS_SORTED, _abi.SHADE_256_GLOBAL_SORTED])
 @pytest.mark.parametrize("trace_blocks,shade_blocks,light_blocks", [(7, 3, 5), (2048, 512, 1024)])
 def test_every_launch_shape_renders_the_same_film(ctx, chart, variant, trace_blocks, shade_blocks, light_blocks):
     _, scene = chart
