@@ -69,7 +69,7 @@ def cpu_baseline(data, width, height, max_bounces, budget_s=12.0):
     on the host's cores, over a bounded band of rows of the same workload."""
     from oracle import oracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = min(16, orc.threads_default())
     scene = orc.OracleScene(data)
     # Probe a band of rows; then trace whole frames at successive sample
     # indices until budget_s has passed (or one band, if a frame would not fit).

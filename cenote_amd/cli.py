@@ -4,11 +4,11 @@
 linear averages as one multi-layer EXR (beauty, albedo and normal guides,
 first-hit depth; output.rs:84-136).
 
-Scenes are the seeded built-ins (``cornell``, ``chart`` — the default, the
-material chart standing in for the reference's built-in demo — and
-``heavy``); their settings fill any flag left out, as the reference's scene
-settings do (main.rs:109-132). Scene files (``.ron``/``.pbrt``) are not read
-by this build.
+The scene is a ``.ron`` scene file (cenote_amd/scenefile.py; its Settings
+fill any flag left out, main.rs:109-132) or one of the seeded built-ins
+(``cornell``, ``chart`` — the default, the material chart standing in for
+the reference's built-in demo — and ``heavy``, with their configurations'
+settings). ``.pbrt`` import is not part of this build.
 """
 
 from __future__ import annotations
@@ -31,7 +31,8 @@ def build_parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser(prog="cenote", description="Cenote: a GPU path tracer (MI355X)")
     sub = ap.add_subparsers(dest="command", required=True)
     r = sub.add_parser("render", help="render a scene to an EXR (the built-in chart when none is named)")
-    r.add_argument("scene", nargs="?", default=None, help="cornell, chart or heavy (default: chart)")
+    r.add_argument("scene", nargs="?", default=None,
+                   help="a .ron scene file, or cornell, chart or heavy (default: chart)")
     r.add_argument("--spp", type=_positive(1), help="samples per pixel (default: the scene's)")
     r.add_argument("--width", type=_positive(1), help="output width in pixels (default: the scene's)")
     r.add_argument("--height", type=_positive(1), help="output height in pixels (default: the scene's)")
@@ -44,18 +45,32 @@ def render(args) -> int:
     import cenote_amd as cn
     from cenote_amd import output, workloads
 
+    from cenote_amd import scenefile
+
     name = args.scene or "chart"
-    if name not in workloads.CONFIGS:
-        print(f"error: unknown scene {name!r} (built-ins: {', '.join(sorted(workloads.CONFIGS))}; "
-              "scene files are not read by this build)", file=sys.stderr)
+    if name.endswith(".ron"):
+        try:
+            data, settings = scenefile.load_scene(name)
+        except (OSError, scenefile.SceneFileError) as e:
+            print(f"error: loading scene {name}: {e}", file=sys.stderr)
+            return 2
+        cfg = {"width": settings["resolution"][0], "height": settings["resolution"][1], "spp": settings["spp"],
+               "max_bounces": settings["max_bounces"]}
+    elif name in workloads.CONFIGS:
+        data, cfg = None, workloads.CONFIGS[name]
+    else:
+        print(f"error: unknown scene {name!r} (a .ron scene file, or a built-in: "
+              f"{', '.join(sorted(workloads.CONFIGS))})", file=sys.stderr)
         return 2
-    cfg = workloads.CONFIGS[name]
     width = args.width or cfg["width"]
     height = args.height or cfg["height"]
     spp = args.spp or cfg["spp"]
     depth = args.depth or cfg["max_bounces"]
+    if not 1 <= depth <= 255:
+        print(f"error: max bounces {depth} outside 1..255", file=sys.stderr)
+        return 2
     ctx = cn.Context(0)
-    scene = cn.Scene(ctx, workloads.build(name))
+    scene = cn.Scene(ctx, data if data is not None else workloads.build(name))
     renderer = cn.Renderer(ctx, max_bounces=depth)
     film = cn.Film(ctx, width, height)
     t0 = time.perf_counter()

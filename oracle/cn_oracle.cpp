@@ -2146,6 +2146,35 @@ void orc_accumulate(orc_scene* scene, uint32_t width, uint32_t height, uint32_t 
     }
 }
 
+void orc_accumulate_rows(orc_scene* scene, uint32_t width, uint32_t height, uint32_t row0, uint32_t row1,
+                         uint32_t first, uint32_t count, uint32_t max_bounces, uint32_t light_sampling,
+                         float* beauty_sum, int threads) {
+    // accumulate.slang:46-67 per pixel of a band of rows: the beauty sum of
+    // samples first .. first+count-1, folded in sample order. Threads own
+    // pixels (not rows), so narrow bands still use every core.
+    float aperture;
+    Basis basis = camera_basis(scene->camera, float(width) / float(height), &aperture);
+    if (threads < 1) threads = 1;
+    const size_t pixels = size_t(row1 - row0) * width;
+    auto worker = [&](int id) {
+        for (size_t k = size_t(id); k < pixels; k += size_t(threads)) {
+            const uint32_t pixel = uint32_t(row0 * width + k);
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t sample = first + i;
+                PixelOut o;
+                trace_path(*scene, basis, aperture, width, height, pixel, sample, max_bounces, light_sampling,
+                           false, o);
+                float c[4] = {o.radiance.x, o.radiance.y, o.radiance.z, o.radiance.w};
+                accumulate4(c, beauty_sum + 4 * k, 0, sample == 0);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+}
+
 void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t sample,
                     uint32_t* instance, uint32_t* primitive, float* bary, int threads) {
     float aperture;

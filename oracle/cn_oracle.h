@@ -85,6 +85,13 @@ void orc_accumulate(orc_scene* scene, uint32_t width, uint32_t height, uint32_t 
                     float* beauty_sum, float* albedo_sum, float* normal_sum, float* depth_sum,
                     int threads);
 
+/* The beauty sums of samples first .. first+count-1 over rows [row0, row1)
+ * of a width x height wave (reset when a pixel's first sample is 0);
+ * threads split the band's pixels. */
+void orc_accumulate_rows(orc_scene* scene, uint32_t width, uint32_t height, uint32_t row0, uint32_t row1,
+                         uint32_t first, uint32_t count, uint32_t max_bounces, uint32_t light_sampling,
+                         float* beauty_sum, int threads);
+
 /* The bounce-0 closest hit per pixel (shaders/intersect.slang:45-88). */
 void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t sample,
                     uint32_t* instance, uint32_t* primitive, float* bary, int threads);

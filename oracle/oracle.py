@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
     L.orc_trace.argtypes = [vp, u, u, u, u, u, C.c_int, fp, fp, fp, fp, C.c_int]
     L.orc_trace_rows.argtypes = [vp, u, u, u, u, u, u, u, C.c_int, fp, fp, fp, fp, C.c_int]
     L.orc_accumulate.argtypes = [vp, u, u, u, u, u, u, fp, fp, fp, fp, C.c_int]
+    L.orc_accumulate_rows.argtypes = [vp, u, u, u, u, u, u, u, u, fp, C.c_int]
     L.orc_first_hits.argtypes = [vp, u, u, u, up, up, fp, C.c_int]
     _lib = L
     return L
@@ -87,7 +88,18 @@ def _fp(a):
 
 
 def threads_default() -> int:
-    return int(os.environ.get("CENOTE_ORACLE_THREADS", os.cpu_count() or 1))
+    """CENOTE_ORACLE_THREADS, else the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when that is set (a host's share of its cores)."""
+    if "CENOTE_ORACLE_THREADS" in os.environ:
+        return max(1, int(os.environ["CENOTE_ORACLE_THREADS"]))
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
 
 
 def sample_2d(pixel: int, sample: int, dimension: int) -> tuple[float, float]:
@@ -191,6 +203,15 @@ class OracleScene:
         b, a, n, d = sums
         lib().orc_accumulate(self.handle, width, height, first, count, max_bounces, light_sampling, _fp(b), _fp(a),
                              _fp(n), _fp(d), threads)
+        return sums
+
+    def accumulate_rows(self, width, height, row0, row1, first, count, max_bounces=8, light_sampling=0,
+                        threads=None):
+        """Beauty sums of samples first..first+count-1 over rows [row0, row1)."""
+        threads = threads or threads_default()
+        sums = np.zeros((row1 - row0, width, 4), np.float32)
+        lib().orc_accumulate_rows(self.handle, width, height, row0, row1, first, count, max_bounces,
+                                  light_sampling, _fp(sums), threads)
         return sums
 
     def first_hits(self, width, height, sample=0, threads=None):

@@ -21,8 +21,10 @@ def test_flags_outside_their_ranges_are_rejected(argv):
 
 
 def test_an_unknown_scene_is_an_error_before_any_device_work(capsys):
-    assert cli.main(["render", "no_such_scene.ron"]) == 2
+    assert cli.main(["render", "no_such_scene"]) == 2
     assert "unknown scene" in capsys.readouterr().err
+    assert cli.main(["render", "no_such_scene.ron"]) == 2
+    assert "No such file" in capsys.readouterr().err
 
 
 def test_defaults_follow_the_reference_flags():
@@ -55,3 +57,33 @@ def test_render_writes_the_film_averages_as_a_multilayer_exr(ctx, tmp_path):
     assert np.array_equal(ch["R"], beauty[..., 0].astype(np.float16))
     assert np.array_equal(ch["albedo.G"], albedo[..., 1].astype(np.float16))
     assert np.array_equal(ch["Z"].view(np.uint32), depth.view(np.uint32))
+
+
+def test_a_broken_scene_file_is_an_error_before_any_device_work(tmp_path, capsys):
+    bad = tmp_path / "bad.ron"
+    bad.write_text("(version: 7, ops: [])")
+    assert cli.main(["render", str(bad)]) == 2
+    assert "version 7" in capsys.readouterr().err
+
+
+@pytest.mark.gpu
+def test_render_reads_a_scene_file(tmp_path):
+    from test_exr import channels
+
+    scene = tmp_path / "tri.ron"
+    scene.write_text('''(version: 1, ops: [
+        Settings((name: "main", resolution: Some((40, 30)), spp: Some(2), max_bounces: Some(3))),
+        Camera((name: "main", position: Some((0.3, 0.3, 2.0)), look_at: Some((0.3, 0.3, 0.0)))),
+        Mesh((name: "tri", source: Some(Inline(positions: [(0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)],
+                                               triangles: [(0, 1, 2)])))),
+        Material((name: "lamp", emission_luminance: Some(2.0))),
+        Instance((name: "t", mesh: Some("tri"), material: Some("lamp"))),
+    ])''')
+    out = tmp_path / "tri.exr"
+    run = subprocess.run([sys.executable, "-m", "cenote_amd.cli", "render", str(scene), "--out", str(out)],
+                         cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    assert "40×30, 2 spp" in run.stdout
+    _, _, ch = channels(out)
+    r = ch["R"].astype(np.float32)
+    assert r.shape == (30, 40) and r.max() > 1.0 and r.min() == 0.0  # the emitter in front of a black sky

@@ -147,3 +147,14 @@ def test_camera_basis_is_the_glam_basis():
     assert np.allclose(fwd, f, atol=1e-6)
     assert np.allclose(right, r * t * 16 / 9, atol=1e-6)
     assert np.allclose(up, u * t, atol=1e-6)
+
+
+def test_band_accumulation_matches_full_frames():
+    """The pixel-parallel band accumulation folds the same samples in the
+    same order as whole-frame accumulation."""
+    from cenote_amd import workloads as wl
+
+    s = orc.OracleScene(wl.cornell())
+    full = s.accumulate(24, 20, 0, 3)[0]
+    band = s.accumulate_rows(24, 20, 5, 12, 0, 3)
+    assert np.array_equal(band, full[5:12])
