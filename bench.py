@@ -151,7 +151,7 @@ def batched_bench(args):
     1..16/9, 1..64 spp): the plain form — view after view, one sample per
     wave from the reference's 2^20-path pool, a synchronous wave each (the
     reference's Renderer::accumulate loop) — against one many-view batch
-    (shared waves from the 2^24-path pool, launch shape from the
+    (shared waves from the 2^26-path pool, launch shape from the
     autotuner). Both forms render bitwise the same films."""
     import cenote_amd as cn
     from cenote_amd import autotune, workloads
@@ -224,7 +224,7 @@ def serving_bench(args):
     specs = serve.poisson_requests({n: d.camera for n, d in data.items()}, depths, args.rate, args.requests)
     log(f"serving: {len(specs)} requests at {args.rate} req/s over {specs[-1].arrival:.2f} s")
     r = cn.Renderer(ctx, max_bounces=16)
-    r.reserve(r.capacity, aov=True)
+    r.reserve(1 << 24, aov=True)  # the server's waves hold at most 2^24 paths
     budget = 8 << 30
     sv = serve.Server(r, budget)
     # Warm: one small request through every scene.

@@ -340,8 +340,8 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     const uint32_t cus = std::max(1u, ctx->compute_units);
     // Defaults from the autotuner's sweeps on the chart and heavy scenes;
     // cn_renderer_set_shape overrides them.
-    r->shape.trace_blocks = cus * 16;
-    r->shape.shade_variant = SHADE_512_LDS;
+    r->shape.trace_blocks = cus * 32;
+    r->shape.shade_variant = SHADE_512_LDS_SORTED;
     r->shape.shade_blocks = cus * 4;
     r->shape.light_blocks = cus * 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));

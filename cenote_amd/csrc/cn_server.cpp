@@ -266,7 +266,7 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
     auto sv = std::make_unique<cn_server>();
     sv->r = renderer;
     sv->budget = hbm_budget_bytes;
-    sv->wave_paths = wave_paths == 0 ? renderer->capacity : std::min(wave_paths, renderer->capacity);
+    sv->wave_paths = std::min(wave_paths == 0 ? (1u << 24) : wave_paths, renderer->capacity);
     sv->t0 = std::chrono::steady_clock::now();
     CN_HIP(sv->partials.alloc(size_t(NOISE_RING) * MAX_JOBS * NOISE_BLOCKS));
     CN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sv->host_partials),

@@ -66,11 +66,11 @@ enum {
 /* Wavefront::DEFAULT_CAPACITY and DEFAULT_MAX_BOUNCES (wavefront.rs:462-471). */
 #define CN_DEFAULT_CAPACITY (1u << 20)
 #define CN_DEFAULT_MAX_BOUNCES 8u
-/* The most paths a capacity of 0 lets a wave hold here: 2^24 (228 bytes of
- * path state and slot targets each, at most 3.8 GB, allocated as views
- * need it), so that one wave carries 8 samples of a 1080p view and even
+/* The most paths a capacity of 0 lets a wave hold here: 2^26 (228 bytes of
+ * path state and slot targets each, at most 15.3 GB, allocated as views
+ * need it), so that one wave carries 32 samples of a 1080p view and even
  * the deep-bounce launches fill the chip. */
-#define CN_POOL_CAPACITY (1u << 24)
+#define CN_POOL_CAPACITY (1u << 26)
 
 /*
  * One instance's lowered OpenPBR surface, 144 bytes, field for field the
@@ -423,7 +423,8 @@ typedef struct cn_server_stats {
     uint64_t hbm_used_bytes;      /* films held by requests */
 } cn_server_stats;
 
-/* wave_paths: most paths per wave (0 = the renderer's capacity). */
+/* wave_paths: most paths per wave (0 = the renderer's capacity, at most
+ * 2^24: smaller waves return first frames sooner). */
 int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t wave_paths,
                      cn_server** out);
 int cn_server_destroy(cn_server* server);
