@@ -102,6 +102,7 @@ struct cn_scene {
     cn::DevBuf<uint32_t> indices;
     cn::DevBuf<float4> env_texels;
     cn::DevBuf<float> env_marginal, env_conditional, env_pdfs;
+    cn::DevBuf<uint32_t> env_marginal_guide, env_conditional_guide;
     std::vector<cn_light_record> host_lights;
     cn::EnvironmentTables host_env;
     cn_scene_info info{};

@@ -89,5 +89,6 @@ hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t d
 hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const CameraView& cam,
                              const PathPool& p, uint32_t width, uint32_t height, uint32_t sample, uint32_t* inst,
                              uint32_t* prim, float* bary);
+hipError_t launch_env_sample(hipStream_t stream, const SceneView& s, const float2* u, uint32_t n, float* out);
 
 }  // namespace cn

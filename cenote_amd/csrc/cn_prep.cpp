@@ -206,7 +206,52 @@ EnvironmentTables build_environment_tables(uint32_t width, uint32_t height, cons
         power += solid * acc;
     }
     t.power = power;
+
+    // A table that does not rise monotonically (non-finite texels) gets one
+    // bucket spanning every index, which is the plain search.
+    auto guides = [&](const float* cdfs, size_t stride, uint32_t count, size_t tables, uint32_t* buckets,
+                      std::vector<uint32_t>* out) {
+        *buckets = guide_buckets(count);
+        out->assign(tables * (*buckets + 1), 0u);
+        std::vector<char> ok(tables, 1);
+        auto one = [&](size_t k) {
+            ok[k] = build_cdf_guide(cdfs + k * stride, count, *buckets, out->data() + k * (*buckets + 1));
+        };
+        if (tables == 1) one(0);
+        else rows_parallel(one);
+        if (std::find(ok.begin(), ok.end(), 0) != ok.end()) {
+            *buckets = 1;
+            out->assign(tables * 2, 0u);
+            for (size_t k = 0; k < tables; ++k) (*out)[2 * k + 1] = count - 1;
+        }
+    };
+    guides(t.marginal.data(), 0, uint32_t(h), 1, &t.marginal_buckets, &t.marginal_guide);
+    guides(t.conditional.data(), w + 1, uint32_t(w), h, &t.conditional_buckets, &t.conditional_guide);
     return t;
+}
+
+uint32_t guide_buckets(uint32_t count) {
+    uint32_t b = 1;
+    while (b < count && b < (1u << 20)) b <<= 1;
+    return b;
+}
+
+static uint32_t cdf_index(const float* cdf, uint32_t count, float u) {
+    uint32_t lo = 0, hi = count;
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+bool build_cdf_guide(const float* cdf, uint32_t count, uint32_t buckets, uint32_t* guide) {
+    bool monotone = cdf[0] <= 0.0f;
+    for (uint32_t i = 1; i < count && monotone; ++i) monotone = cdf[i - 1] <= cdf[i];
+    if (!monotone) return false;
+    for (uint32_t g = 0; g <= buckets; ++g) guide[g] = cdf_index(cdf, count, float(g) / float(buckets));
+    return true;
 }
 
 float select_probability(double env_power, double light_power) {

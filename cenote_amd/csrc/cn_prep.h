@@ -40,10 +40,25 @@ struct EnvironmentTables {
     std::vector<float> conditional;  // height * (width + 1)
     std::vector<float> pdfs;         // height * width
     double power = 0.0;
+    // Search guides for the CDFs: marginal_buckets + 1 entries, then
+    // conditional_buckets + 1 per row (see build_cdf_guide).
+    std::vector<uint32_t> marginal_guide, conditional_guide;
+    uint32_t marginal_buckets = 1, conditional_buckets = 1;
 };
 
 // environment.rs:130-218
 EnvironmentTables build_environment_tables(uint32_t width, uint32_t height, const float* texels);
+
+// The index the CDF inversion of environment.slang:41-58 returns for u is
+// the last i in [0, count) with cdf[i] <= u (cdf[0] = 0); it never decreases
+// as u grows. The guide holds that index at u = g / buckets for g in
+// 0..buckets (buckets a power of two, so u * buckets and g / buckets are
+// exact): for u in bucket g the answer lies in [guide[g], guide[g + 1]], and
+// a binary search over that range returns the same index as one over the
+// whole table. Returns false, writing nothing, for a table that does not
+// rise monotonically.
+uint32_t guide_buckets(uint32_t count);
+bool build_cdf_guide(const float* cdf, uint32_t count, uint32_t buckets, uint32_t* guide);
 
 // scene/mod.rs:665-671
 float select_probability(double env_power, double light_power);

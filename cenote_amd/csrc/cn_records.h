@@ -20,7 +20,7 @@ enum Queue : uint32_t { Q_RAY = 0, Q_HIT = 1, Q_MISS = 2, Q_SHADOW = 3, Q_COUNT 
 
 // Most pixel ranges a single wave may walk (pixels / capacity, rounded up).
 constexpr uint32_t MAX_RANGES = 64;
-// Counters one range holds: 4 queues per bounce, plus one spare bounce.
+// Counters one range holds: Q_COUNT per bounce, plus one spare bounce.
 constexpr uint32_t counters_per_range(uint32_t max_bounces) { return (max_bounces + 1) * Q_COUNT; }
 
 // Floats in the BSDF table blob (crates/cenote/src/tables.rs:67-80).
@@ -107,10 +107,14 @@ struct SceneView {
     const float* env_marginal;
     const float* env_conditional;
     const float* env_pdfs;
+    const uint32_t* env_marginal_guide;     // env_marginal_buckets + 1 entries
+    const uint32_t* env_conditional_guide;  // env_conditional_buckets + 1 per row
     uint32_t light_count;
     uint32_t env_width;
     uint32_t env_height;
     float env_select_prob;
+    uint32_t env_marginal_buckets;
+    uint32_t env_conditional_buckets;
 };
 
 // The SoA path pool (shaders/pathstate.slang:207-232). A wave is a set of

@@ -253,6 +253,10 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     CN_HIP(scene->env_marginal.upload(scene->host_env.marginal.data(), scene->host_env.marginal.size()));
     CN_HIP(scene->env_conditional.upload(scene->host_env.conditional.data(), scene->host_env.conditional.size()));
     CN_HIP(scene->env_pdfs.upload(scene->host_env.pdfs.data(), scene->host_env.pdfs.size()));
+    CN_HIP(scene->env_marginal_guide.upload(scene->host_env.marginal_guide.data(),
+                                            scene->host_env.marginal_guide.size()));
+    CN_HIP(scene->env_conditional_guide.upload(scene->host_env.conditional_guide.data(),
+                                               scene->host_env.conditional_guide.size()));
 
     SceneView& v = scene->view;
     v.nodes = scene->nodes.ptr;
@@ -268,6 +272,10 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     v.env_marginal = scene->env_marginal.ptr;
     v.env_conditional = scene->env_conditional.ptr;
     v.env_pdfs = scene->env_pdfs.ptr;
+    v.env_marginal_guide = scene->env_marginal_guide.ptr;
+    v.env_conditional_guide = scene->env_conditional_guide.ptr;
+    v.env_marginal_buckets = scene->host_env.marginal_buckets;
+    v.env_conditional_buckets = scene->host_env.conditional_buckets;
     v.light_count = uint32_t(scene->host_lights.size());
     v.env_width = desc->environment.width;
     v.env_height = desc->environment.height;
@@ -348,6 +356,20 @@ int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional, fl
     if (marginal) std::memcpy(marginal, t.marginal.data(), t.marginal.size() * 4);
     if (conditional) std::memcpy(conditional, t.conditional.data(), t.conditional.size() * 4);
     if (pdfs) std::memcpy(pdfs, t.pdfs.data(), t.pdfs.size() * 4);
+    return CN_OK;
+}
+
+int cn_scene_sample_environment(cn_scene* scene, const float* u, uint32_t count, float* out) {
+    if (!scene || (count && (!u || !out))) return fail(CN_E_INVALID, "cn_scene_sample_environment: null argument");
+    if (count == 0) return CN_OK;
+    CN_HIP(hipSetDevice(scene->ctx->device));
+    DevBuf<float2> du;
+    DevBuf<float> dout;
+    CN_HIP(du.upload(reinterpret_cast<const float2*>(u), count));
+    CN_HIP(dout.alloc(size_t(count) * 12));
+    CN_HIP(launch_env_sample(nullptr, scene->view, du.ptr, count, dout.ptr));
+    CN_HIP(hipDeviceSynchronize());
+    CN_HIP(hipMemcpy(out, dout.ptr, size_t(count) * 12 * 4, hipMemcpyDeviceToHost));
     return CN_OK;
 }
 

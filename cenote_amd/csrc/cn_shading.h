@@ -629,10 +629,17 @@ __device__ __forceinline__ v3 env_radiance(const SceneView& s, v3 d) {
     return env_bilinear(s, uv.x, uv.y);
 }
 
-// environment.slang:41-58
-__device__ __forceinline__ void sample_cdf(const float* __restrict__ cdf, uint32_t count, float u, uint32_t* index,
+// environment.slang:41-58. The index is the last i in [0, count) with
+// cdf[i] <= u; the guide (buckets + 1 entries, buckets a power of two) holds
+// that index at u = g / buckets, so for u in bucket g it lies in
+// [guide[g], guide[g + 1]] and the search covers only that range — the same
+// index the search over the whole table finds, in a few steps instead of
+// log2(count) dependent loads.
+__device__ __forceinline__ void sample_cdf(const float* __restrict__ cdf, const uint32_t* __restrict__ guide,
+                                           uint32_t buckets, uint32_t count, float u, uint32_t* index,
                                            float* position) {
-    uint32_t lo = 0, hi = count;
+    const uint32_t g = min(uint32_t(u * float(buckets)), buckets - 1);
+    uint32_t lo = guide[g], hi = guide[g + 1] + 1;
     while (lo + 1 < hi) {
         const uint32_t mid = (lo + hi) / 2;
         if (cdf[mid] <= u) lo = mid;
@@ -653,8 +660,10 @@ struct EnvSample {
 __device__ inline EnvSample env_sample(const SceneView& s, f2 u) {
     uint32_t row, col;
     float v, uu;
-    sample_cdf(s.env_marginal, s.env_height, u.x, &row, &v);
-    sample_cdf(s.env_conditional + row * (s.env_width + 1), s.env_width, u.y, &col, &uu);
+    sample_cdf(s.env_marginal, s.env_marginal_guide, s.env_marginal_buckets, s.env_height, u.x, &row, &v);
+    sample_cdf(s.env_conditional + row * (s.env_width + 1),
+               s.env_conditional_guide + row * (s.env_conditional_buckets + 1), s.env_conditional_buckets,
+               s.env_width, u.y, &col, &uu);
     const float sinTheta = sinf(PI_K * v);
     const float phi = 2.0f * PI_K * (uu - 0.5f);
     const float theta = PI_K * v;

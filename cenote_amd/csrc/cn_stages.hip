@@ -1,4 +1,4 @@
-// cn_kernels.hip — the wavefront stage kernels for gfx950.
+// cn_stages.hip — the wavefront stage kernels for gfx950.
 //
 // One wave traces one sample of every pixel of a target
 // (crates/cenote/src/wavefront.rs:1-28): raygen, then per bounce intersect ->
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool
     const uint32_t n = n_ray + n_shadow;
     const uint32_t* rays = p.ray_queue[bounce & 1];
     const uint32_t lanes = gridDim.x * BLOCK;
-    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x);
     const uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
     uint32_t ray_nodes = 0, ray_tris = 0, shadow_nodes = 0, shadow_tris = 0;
     for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_hits(SceneView s, CameraView ca
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
     const uint32_t n = width * height;
     const uint32_t lanes = gridDim.x * BLOCK;
-    Stack st{lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x, 0};
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x);
     uint32_t nodes = 0, tris = 0;
     for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
         const uint32_t pixel = first + threadIdx.x;
@@ -669,6 +669,36 @@ __global__ __launch_bounds__(BLOCK) void k_first_hits(SceneView s, CameraView ca
         prim_out[pixel] = h.hit ? h.prim : 0xffffffffu;
         bary_out[2 * pixel] = h.hit ? h.b1 : 0.0f;
         bary_out[2 * pixel + 1] = h.hit ? h.b2 : 0.0f;
+    }
+}
+
+// Environment importance sampling for given uniforms (the CDF inversion
+// check): per sample, the direction, pdf and radiance of env_sample and the
+// (v, u) position and (row, column) the two inversions chose.
+__global__ __launch_bounds__(BLOCK) void k_env_sample(SceneView s, const float2* __restrict__ u, uint32_t n,
+                                                      float* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const float2 ui = u[i];
+        uint32_t row, col;
+        float v, uu;
+        sample_cdf(s.env_marginal, s.env_marginal_guide, s.env_marginal_buckets, s.env_height, ui.x, &row, &v);
+        sample_cdf(s.env_conditional + row * (s.env_width + 1),
+                   s.env_conditional_guide + row * (s.env_conditional_buckets + 1), s.env_conditional_buckets,
+                   s.env_width, ui.y, &col, &uu);
+        const EnvSample e = env_sample(s, {ui.x, ui.y});
+        float* o = out + 12 * size_t(i);
+        o[0] = e.direction.x;
+        o[1] = e.direction.y;
+        o[2] = e.direction.z;
+        o[3] = e.pdf;
+        o[4] = e.radiance.x;
+        o[5] = e.radiance.y;
+        o[6] = e.radiance.z;
+        o[7] = v;
+        o[8] = uu;
+        o[9] = __uint_as_float(row);
+        o[10] = __uint_as_float(col);
+        o[11] = 0.0f;
     }
 }
 
@@ -761,6 +791,11 @@ hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const
                              uint32_t* prim, float* bary) {
     hipLaunchKernelGGL(k_first_hits, dim3(shape.trace_blocks), dim3(BLOCK), 0, stream, s, cam, p, width, height, sample,
                        inst, prim, bary);
+    return hipGetLastError();
+}
+
+hipError_t launch_env_sample(hipStream_t stream, const SceneView& s, const float2* u, uint32_t n, float* out) {
+    hipLaunchKernelGGL(k_env_sample, dim3(grid_for(n, 1024)), dim3(BLOCK), 0, stream, s, u, n, out);
     return hipGetLastError();
 }
 

@@ -26,10 +26,17 @@ constexpr uint32_t NODE_DONE = CHILD_EMPTY;  // never pushed: empty slots fail e
 // the rest in a per-lane spill area in HBM laid out [entry][lane] so spills
 // coalesce. The builder rejects trees that could need more than STACK_MAX
 // entries; a push past it is dropped all the same, so every pop stays in
-// bounds.
+// bounds. The LDS part is typed as LDS (address space 3) and the spill as
+// global memory (address space 1), so each access is a ds_* or global_*
+// instruction; through plain pointers the compiler merged the two loads of
+// a pop into one flat load, which waits on every outstanding global load
+// too.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
 struct Stack {
-    uint32_t* lds;
-    uint32_t* spill;
+    lds_u32* lds;
+    global_u32* spill;
     uint32_t lanes;
     uint32_t lane;
     uint32_t sp;
@@ -39,12 +46,36 @@ struct Stack {
         else spill[(sp - STACK_LDS) * lanes + lane] = v;
         ++sp;
     }
+    // Pushes the hit children after the nearest, farthest first: c3, c2,
+    // c1 for nh = 4, c2, c1 for 3, c1 for 2. Away from the LDS boundary the
+    // three stores go out unconditionally; a store past the new top lands
+    // above it, where nothing reads.
+    __device__ __forceinline__ void push_sorted(uint32_t c3, uint32_t c2, uint32_t c1, uint32_t nh) {
+        if (sp + 3 <= STACK_LDS) {
+            uint32_t at = sp;
+            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c3;
+            at += nh > 3 ? 1u : 0u;
+            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c2;
+            at += nh > 2 ? 1u : 0u;
+            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c1;
+            at += nh > 1 ? 1u : 0u;
+            sp = at;
+            return;
+        }
+        if (nh > 3) push(c3);
+        if (nh > 2) push(c2);
+        if (nh > 1) push(c1);
+    }
     __device__ __forceinline__ uint32_t pop_or_done() {
         if (sp == 0) return NODE_DONE;
         --sp;
         return sp < STACK_LDS ? lds[sp * TRAVERSAL_BLOCK + threadIdx.x] : spill[(sp - STACK_LDS) * lanes + lane];
     }
 };
+
+__device__ __forceinline__ Stack make_stack(uint32_t* lds_stack, uint32_t* spill, uint32_t lanes, uint32_t lane) {
+    return Stack{(lds_u32*)lds_stack, (global_u32*)spill, lanes, lane, 0};
+}
 
 __device__ __forceinline__ float safe_rcp(float x) {
     return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x);
@@ -135,8 +166,23 @@ __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
     return r;
 }
 
+// One compare-exchange of a stable sort by key: adjacent positions only,
+// swapping on strictly greater, so equal keys keep their slot order.
+__device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, uint32_t& cb) {
+    const bool swap = ka > kb;
+    const float k = swap ? kb : ka;
+    const uint32_t c = swap ? cb : ca;
+    kb = swap ? ka : kb;
+    cb = swap ? ca : cb;
+    ka = k;
+    ca = c;
+}
+
 // Box-test the four children of one node against [0, bound]; push the hit
 // ones farthest first and return the nearest (or the next stack entry).
+// Hit children are ordered by entry distance, ties in slot order: an
+// odd-even transposition network (six adjacent compare-exchanges, no
+// branches), misses keyed +inf so they sort last.
 __device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                                float bound, Stack& st) {
     const BvhNode4 n = nodes[index];
@@ -147,9 +193,9 @@ __device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ node
     const float lz[4] = {n.lo_z.x, n.lo_z.y, n.lo_z.z, n.lo_z.w};
     const float hz[4] = {n.hi_z.x, n.hi_z.y, n.hi_z.z, n.hi_z.w};
     const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-    uint32_t hitc[4];
-    float hitd[4];
-    int nh = 0;
+    float key[4];
+    uint32_t ref[4];
+    uint32_t nh = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const float ax = fmaf(lx[c], r.ix, -r.ox), bx = fmaf(hx[c], r.ix, -r.ox);
@@ -157,20 +203,20 @@ __device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ node
         const float az = fmaf(lz[c], r.iz, -r.oz), bz = fmaf(hz[c], r.iz, -r.oz);
         const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
         const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound)) * 1.0000004f;
-        if (t0 <= t1 && ch[c] != CHILD_EMPTY) {
-            int k = nh++;
-            while (k > 0 && hitd[k - 1] > t0) {
-                hitd[k] = hitd[k - 1];
-                hitc[k] = hitc[k - 1];
-                --k;
-            }
-            hitd[k] = t0;
-            hitc[k] = ch[c];
-        }
+        const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
+        key[c] = hit ? t0 : __int_as_float(0x7f800000);
+        ref[c] = hit ? ch[c] : NODE_DONE;
+        nh += hit ? 1u : 0u;
     }
+    order_pair(key[0], ref[0], key[1], ref[1]);
+    order_pair(key[2], ref[2], key[3], ref[3]);
+    order_pair(key[1], ref[1], key[2], ref[2]);
+    order_pair(key[0], ref[0], key[1], ref[1]);
+    order_pair(key[2], ref[2], key[3], ref[3]);
+    order_pair(key[1], ref[1], key[2], ref[2]);
     if (nh == 0) return st.pop_or_done();
-    for (int k = nh - 1; k > 0; --k) st.push(hitc[k]);
-    return hitc[0];
+    st.push_sorted(ref[3], ref[2], ref[1], nh);
+    return ref[0];
 }
 
 // The speculative while-while walk over [0, bound]. `leaf(ref)` tests one
@@ -202,84 +248,6 @@ __device__ __forceinline__ void walk(const BvhNode4* __restrict__ nodes, const S
         if (cur == NODE_DONE) return;
     }
 }
-
-// A resumable walk for persistent lanes: the node to descend next and the
-// postponed leaf. The ray is done when both are NODE_DONE.
-struct WalkState {
-    uint32_t cur, leaf;
-    __device__ __forceinline__ bool done() const { return cur == NODE_DONE && leaf == NODE_DONE; }
-};
-
-// One while-while round of `walk`: descend until every lane still
-// descending holds a postponed leaf, then test leaves until the next one is
-// an internal node.
-template <typename Leaf>
-__device__ __forceinline__ void walk_round(const BvhNode4* __restrict__ nodes, const SlabRay& r, float& bound,
-                                           Stack& st, WalkState& ws, uint32_t& nodes_visited, Leaf&& leaf_fn) {
-    uint32_t cur = ws.cur, leaf = ws.leaf;
-    while (!(cur & CHILD_LEAF)) {
-        cur = visit_node(nodes, cur, r, bound, st);
-        ++nodes_visited;
-        if ((cur & CHILD_LEAF) && cur != NODE_DONE && leaf == NODE_DONE) {
-            leaf = cur;
-            cur = st.pop_or_done();
-        }
-        if (!__any(leaf == NODE_DONE)) break;
-    }
-    while (leaf != NODE_DONE) {
-        if (leaf_fn(leaf)) {
-            cur = NODE_DONE;
-            leaf = NODE_DONE;
-            break;
-        }
-        leaf = NODE_DONE;
-        if ((cur & CHILD_LEAF) && cur != NODE_DONE) {
-            leaf = cur;
-            cur = st.pop_or_done();
-        }
-    }
-    ws.cur = cur;
-    ws.leaf = leaf;
-}
-
-// Wave-private work distribution for persistent lanes: the wave claims
-// chunks of queue indices with one atomic each and hands them to its idle
-// lanes, so a long ray no longer holds 63 finished lanes hostage.
-struct WaveFetch {
-    uint32_t* counter;
-    uint32_t total;
-    uint32_t next = 0, end = 0;
-    bool exhausted = false;
-    static constexpr uint32_t CHUNK = 64;
-    // For every lane with `idle` set: an index into the queue, or
-    // 0xffffffff when the queue is drained (the lane is then finished for
-    // good); lanes that get neither wait for the next call.
-    __device__ __forceinline__ uint32_t take(bool idle, bool* drained) {
-        *drained = false;
-        const uint64_t need = __ballot(idle);
-        if (need == 0) return 0xffffffffu;
-        const uint32_t lane = __lane_id();
-        if (next >= end && !exhausted) {
-            const uint32_t leader = __ffsll((unsigned long long)need) - 1;
-            uint32_t c = 0;
-            if (lane == leader) c = atomicAdd(counter, CHUNK);
-            c = __shfl(c, leader);
-            next = c;
-            end = c + CHUNK < total ? c + CHUNK : total;
-            if (c + CHUNK >= total) exhausted = true;
-        }
-        const uint32_t avail = end > next ? end - next : 0;
-        const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
-        const uint32_t granted = __popcll(need) < avail ? __popcll(need) : avail;
-        uint32_t index = 0xffffffffu;
-        if (idle) {
-            if (rank < avail) index = next + rank;
-            else if (exhausted) *drained = true;
-        }
-        next += granted;
-        return index;
-    }
-};
 
 __device__ __forceinline__ void leaf_range(uint32_t ref, uint32_t* first, uint32_t* count) {
     *first = ref & LEAF_FIRST_MASK;

@@ -118,6 +118,16 @@ class Scene:
         check(self._lib.cn_scene_env_tables(self.handle, _fp(marginal), _fp(conditional), _fp(pdfs)))
         return marginal, conditional, pdfs
 
+    def sample_environment(self, u: np.ndarray):
+        """The device's environment draw for uniform pairs ``u`` (n x 2):
+        (direction n x 3, pdf n, radiance n x 3, position n x 2 as (v, u),
+        row n, column n)."""
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 12), dtype=np.float32)
+        check(self._lib.cn_scene_sample_environment(self.handle, _fp(u), len(u), _fp(out)))
+        idx = out[:, 9:11].view(np.uint32)
+        return out[:, 0:3], out[:, 3], out[:, 4:7], out[:, 7:9], idx[:, 0].copy(), idx[:, 1].copy()
+
     def close(self):
         if self.handle:
             check(self._lib.cn_scene_destroy(self.handle))

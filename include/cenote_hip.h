@@ -276,6 +276,12 @@ int cn_scene_light_records(cn_scene* scene, cn_light_record* out, uint32_t capac
                            uint32_t* count);
 int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional,
                         float* pdfs);
+/* Environment importance sampling on the device for `count` uniform pairs
+ * (u: 2 floats each) — Environment::sample (environment.slang:90-104), the
+ * draw shade_surface's next-event estimation makes. out: 12 floats per
+ * pair: direction xyz, pdf, radiance rgb, the inverted (v, u) position, and
+ * the chosen row and column as u32 bits (the last float is zero). */
+int cn_scene_sample_environment(cn_scene* scene, const float* u, uint32_t count, float* out);
 
 /* Wavefront::new / Renderer::with_max_bounces (wavefront.rs:486-540,
  * render/mod.rs:151): capacity paths in the pool (0 = CN_POOL_CAPACITY),

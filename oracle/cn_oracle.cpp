@@ -2234,4 +2234,21 @@ void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t 
     for (auto& th : pool) th.join();
 }
 
+void orc_env_sample(orc_scene* scene, const float* u, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const f2 ui{u[2 * i], u[2 * i + 1]};
+        const CdfSlot row = sampleCdf(scene->env.marginal.data(), scene->env_h, ui.x);
+        const CdfSlot col =
+            sampleCdf(scene->env.conditional.data() + row.index * (scene->env_w + 1), scene->env_w, ui.y);
+        const EnvSample e = env_sample(*scene, ui);
+        float* o = out + 12 * size_t(i);
+        const float vals[9] = {e.direction.x, e.direction.y, e.direction.z, e.pdf, e.radiance.x,
+                               e.radiance.y, e.radiance.z, row.position, col.position};
+        std::memcpy(o, vals, sizeof vals);
+        std::memcpy(o + 9, &row.index, 4);
+        std::memcpy(o + 10, &col.index, 4);
+        o[11] = 0.0f;
+    }
+}
+
 }  // extern "C"

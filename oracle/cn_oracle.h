@@ -96,6 +96,11 @@ void orc_accumulate_rows(orc_scene* scene, uint32_t width, uint32_t height, uint
 void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t sample,
                     uint32_t* instance, uint32_t* primitive, float* bary, int threads);
 
+/* Environment::sample (environment.slang:90-104) for `count` uniform pairs:
+ * 12 floats per pair — direction xyz, pdf, radiance rgb, the inverted
+ * (v, u) position, the row and column as u32 bits, zero. */
+void orc_env_sample(orc_scene* scene, const float* u, uint32_t count, float* out);
+
 #ifdef __cplusplus
 }
 #endif

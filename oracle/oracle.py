@@ -79,6 +79,7 @@ def lib() -> C.CDLL:
     L.orc_accumulate.argtypes = [vp, u, u, u, u, u, u, fp, fp, fp, fp, C.c_int]
     L.orc_accumulate_rows.argtypes = [vp, u, u, u, u, u, u, u, u, fp, C.c_int]
     L.orc_first_hits.argtypes = [vp, u, u, u, up, up, fp, C.c_int]
+    L.orc_env_sample.argtypes = [vp, fp, u, fp]
     _lib = L
     return L
 
@@ -213,6 +214,15 @@ class OracleScene:
         lib().orc_accumulate_rows(self.handle, width, height, row0, row1, first, count, max_bounces,
                                   light_sampling, _fp(sums), threads)
         return sums
+
+    def sample_environment(self, u: np.ndarray):
+        """Environment::sample for uniform pairs ``u`` (n x 2), laid out as
+        Scene.sample_environment returns it."""
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 12), dtype=np.float32)
+        lib().orc_env_sample(self.handle, _fp(u), len(u), _fp(out))
+        idx = out[:, 9:11].view(np.uint32)
+        return out[:, 0:3], out[:, 3], out[:, 4:7], out[:, 7:9], idx[:, 0].copy(), idx[:, 1].copy()
 
     def first_hits(self, width, height, sample=0, threads=None):
         threads = threads or threads_default()

@@ -72,6 +72,71 @@ def test_environment_tables_match_oracle_bitwise(chart_pair):
     assert scene.info().env_power == power
 
 
+def _env_uniforms(marginal, conditional, width, height, seed=11):
+    """Uniform pairs that probe the CDF inversion where it can go wrong:
+    the table values themselves and their float neighbours, bucket edges
+    g / 2^k, 0 and 1, and random pairs."""
+    rng = np.random.default_rng(seed)
+    one = np.float32(1.0)
+    edges = np.concatenate([marginal, np.nextafter(marginal, np.float32(0)), np.nextafter(marginal, one),
+                            np.arange(2 * height + 1, dtype=np.float32) / np.float32(2 * height)])
+    edges = np.clip(edges, 0, 1).astype(np.float32)
+    cond = conditional.reshape(height, width + 1)
+    pairs = [np.stack([edges, rng.random(len(edges), dtype=np.float32)], axis=1)]
+    for row in rng.choice(height, size=min(height, 48), replace=False):
+        ux = np.float32((np.float64(marginal[row]) + np.float64(marginal[row + 1])) / 2)
+        ys = np.concatenate([cond[row], np.nextafter(cond[row], np.float32(0)), np.nextafter(cond[row], one),
+                             np.arange(2 * width + 1, dtype=np.float32) / np.float32(2 * width)])
+        ys = np.clip(ys, 0, 1).astype(np.float32)
+        pairs.append(np.stack([np.full(len(ys), ux, np.float32), ys], axis=1))
+    pairs.append(rng.random((20000, 2), dtype=np.float32))
+    pairs.append(np.array([[0, 0], [1, 1], [0, 1], [1, 0]], np.float32))
+    return np.concatenate(pairs).astype(np.float32)
+
+
+def _same_environment_draws(scene, ref, u):
+    d, pdf, rad, pos, row, col = scene.sample_environment(u)
+    rd, rpdf, rrad, rpos, rrow, rcol = ref.sample_environment(u)
+    assert np.array_equal(row, rrow) and np.array_equal(col, rcol)
+    assert np.array_equal(pos.view(np.uint32), rpos.view(np.uint32))
+    # Direction and pdf go through sin/cos, whose last place may differ.
+    ok = np.isfinite(rd).all(axis=1)
+    assert np.allclose(d[ok], rd[ok], rtol=0, atol=2e-6)
+    assert np.allclose(pdf[ok], rpdf[ok], rtol=1e-5, atol=0)
+    assert np.allclose(rad[ok], rrad[ok], rtol=1e-5, atol=0)
+
+
+def test_environment_draws_match_oracle(chart_pair):
+    """The device inverts the CDFs through its search guides; the index it
+    finds must be the whole-table search's (environment.slang:41-58)."""
+    data, scene, ref = chart_pair
+    marginal, conditional, _ = scene.env_tables()
+    info = scene.info()
+    u = _env_uniforms(marginal, conditional, info.env_width, info.env_height)
+    _same_environment_draws(scene, ref, u)
+
+
+def test_environment_draws_with_non_finite_texels(ctx):
+    """A texel of +inf turns the tables into NaN; the CDFs no longer rise,
+    the guides give way to the whole-table search, and the draws still
+    match the oracle's index for index."""
+    data = workloads.cornell()
+    texels = workloads.sky(64, 32)
+    texels[5, 9, :3] = np.inf
+    data.env_texels = texels
+    scene = cn.Scene(ctx, data)
+    try:
+        ref = orc.OracleScene(data)
+        marginal, conditional, _ = scene.env_tables()
+        u = _env_uniforms(np.nan_to_num(marginal), np.nan_to_num(conditional), 64, 32)
+        d, pdf, rad, pos, row, col = scene.sample_environment(u)
+        rd, rpdf, rrad, rpos, rrow, rcol = ref.sample_environment(u)
+        assert np.array_equal(row, rrow) and np.array_equal(col, rcol)
+        assert np.array_equal(pos.view(np.uint32), rpos.view(np.uint32))
+    finally:
+        scene.close()
+
+
 # --- First hits (intersect.slang:45-88 vs the oracle's watertight test) -----------
 
 def _first_hit_agreement(ctx, data, scene, ref, w, h, sample=0):
