@@ -329,6 +329,8 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
                        cn_renderer** out) {
     if (!ctx || !out) return fail(CN_E_INVALID, "cn_renderer_create: null argument");
     if (capacity == 0) capacity = CN_POOL_CAPACITY;
+    if (capacity > CN_POOL_CAPACITY_MAX)
+        return fail(CN_E_INVALID, "capacity %u above the %u-path maximum", capacity, CN_POOL_CAPACITY_MAX);
     if (max_bounces == 0 || max_bounces > 255) return fail(CN_E_INVALID, "max_bounces %u outside 1..255", max_bounces);
     if (light_sampling > CN_LIGHT_SAMPLING_NEE) return fail(CN_E_INVALID, "light_sampling %u", light_sampling);
     CN_HIP(hipSetDevice(ctx->device));
@@ -341,8 +343,8 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     // Defaults from the autotuner's sweeps on the chart and heavy scenes;
     // cn_renderer_set_shape overrides them.
     r->shape.trace_blocks = cus * 32;
-    r->shape.shade_variant = SHADE_512_LDS_SORTED;
-    r->shape.shade_blocks = cus * 4;
+    r->shape.shade_variant = SHADE_256_GLOBAL_SORTED;
+    r->shape.shade_blocks = cus * 8;
     r->shape.light_blocks = cus * 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * 256 * STACK_SPILL));

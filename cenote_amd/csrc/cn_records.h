@@ -65,13 +65,22 @@ __host__ __device__ inline uint32_t leaf_encode(uint32_t first, uint32_t count) 
 
 // One triangle in leaf order, world space, 48 bytes:
 //   v0.w = instance (bits), v1.w = primitive (bits),
-//   v2.w = flags (bits): bit 0 set = non-opaque instance; bits 8..15 = mask.
+//   v2.w = flags (bits): bit 0 set = non-opaque instance; bits 8..15 = mask;
+//          bits 16..20 = the material class (which lobes the closure has).
 struct alignas(16) BvhTriangle {
     float4 v0, v1, v2;
 };
 static_assert(sizeof(BvhTriangle) == 48, "three dwordx4 fetches per triangle");
 
 constexpr uint32_t TRI_NON_OPAQUE = 1u;
+constexpr uint32_t TRI_CLASS_SHIFT = 16;  // bits 16..20: the instance material's class
+
+// A hit-queue entry carries the path index in its low HIT_PATH_BITS bits and
+// the hit's material class above them, so shading can order hits by class
+// without a dependent load. Pools hold at most CN_POOL_CAPACITY_MAX paths.
+constexpr uint32_t HIT_PATH_BITS = 27;
+constexpr uint32_t HIT_PATH_MASK = (1u << HIT_PATH_BITS) - 1;
+static_assert(CN_POOL_CAPACITY_MAX <= (1u << HIT_PATH_BITS), "path indices fit a hit-queue entry");
 
 // --- Scene records -----------------------------------------------------------
 

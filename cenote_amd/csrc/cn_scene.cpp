@@ -198,7 +198,8 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
         // light, in primitive order; the record points at the first.
         const bool emissive = emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f;
         if (emissive && mesh.triangle_count > 0) r.light = uint32_t(emitters.size());
-        const uint32_t tri_flags = (mask << 8) | (opaque ? 0u : TRI_NON_OPAQUE);
+        const uint32_t tri_flags =
+            (r.pad[0] << TRI_CLASS_SHIFT) | (mask << 8) | (opaque ? 0u : TRI_NON_OPAQUE);
         for (uint32_t p = 0; p < mesh.triangle_count; ++p) {
             v3 w[3];
             for (int k = 0; k < 3; ++k) {

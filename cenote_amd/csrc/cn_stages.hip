@@ -27,11 +27,13 @@ namespace cn {
 
 constexpr uint32_t BLOCK = 256;
 constexpr uint32_t ROULETTE_START = 3;   // shade_surface.slang:53
+constexpr uint32_t MATERIAL_CLASSES = 32;  // material_class() values: 5 bits
 
 // --- wave64 queue aggregation ------------------------------------------------
 
 // Claim a slot in `counter` for every lane whose `active` is set, with one
-// atomic per wave. Must be reached by every lane of the wave.
+// atomic per wave. Works in divergent code: the ballot sees only the lanes
+// executing the call, and the leader is one of them.
 __device__ __forceinline__ uint32_t wave_claim(uint32_t* counter, bool active) {
     const uint64_t mask = __ballot(active);
     if (mask == 0) return 0;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool
     for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
         const uint32_t i = first + threadIdx.x;
         const bool is_ray = i < n_ray;
-        uint32_t path = 0;
+        uint32_t path = 0, cls = 0;
         bool hit = false;
         if (is_ray) {
             path = bounce == 0 ? i : rays[i];
@@ -175,6 +177,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool
             const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+            cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
         } else if (i < n) {
             const uint32_t j = i - n_ray;
             const float4 o4 = p.shadow_origin[j];
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool
             const uint32_t hs = wave_claim(&counters[bounce * Q_COUNT + Q_HIT], is_ray && hit);
             const uint32_t ms = wave_claim(&counters[bounce * Q_COUNT + Q_MISS], is_ray && !hit);
             if (is_ray) {
-                if (hit) p.hit_queue[hs] = path;
+                if (hit) p.hit_queue[hs] = path | (cls << HIT_PATH_BITS);
                 else p.miss_queue[ms] = path;
             }
         }
@@ -315,8 +318,6 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
 // can instead of every branch of the closure. The order in which a
 // workgroup shades its hits reaches no result: every add lands in the
 // path's own slot.
-constexpr uint32_t MATERIAL_CLASSES = 32;
-
 template <uint32_t SB, bool LDS_TABLES, bool SORT>
 __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
@@ -337,12 +338,13 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
     for (uint32_t first = blockIdx.x * SB; first < n; first += gridDim.x * SB) {
         const uint32_t i = first + threadIdx.x;
         bool valid = i < n;
-        uint32_t path = valid ? p.hit_queue[i] : 0u;
+        const uint32_t entry = valid ? p.hit_queue[i] : 0u;
+        uint32_t path = entry & HIT_PATH_MASK;
         if constexpr (SORT) {
             // Bucket MATERIAL_CLASSES holds the lanes past the queue's end.
             __shared__ uint32_t bucket[MATERIAL_CLASSES + 1];
             __shared__ uint32_t sorted[SB];
-            const uint32_t cls = valid ? s.instances[p.hit[path].x].pad[0] : MATERIAL_CLASSES;
+            const uint32_t cls = valid ? entry >> HIT_PATH_BITS : MATERIAL_CLASSES;
             if (threadIdx.x <= MATERIAL_CLASSES) bucket[threadIdx.x] = 0;
             __syncthreads();
             const uint32_t rank = atomicAdd(&bucket[cls], 1u);
@@ -362,11 +364,8 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             valid = path != 0xffffffffu;
             if (!valid) path = 0;
         }
-        bool push_ray = false, push_shadow = false;
+        bool push_ray = false;
         uint32_t pixel = 0;
-        v3 sh_origin = S3(0.0f), sh_dir = S3(0.0f), sh_contrib = S3(0.0f);
-        float sh_tmax = 0.0f;
-        uint32_t sh_light = LIGHT_NONE, sh_prim = 0;
         if (valid) {
             pixel = base + path;  // the path's target slot
             const uint2 key = p.key[path];
@@ -411,6 +410,35 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                 }
                 contribution = (T * material.emission) * weight;
             }
+            // The emission lands now rather than when the path's fate is
+            // known; a terminal path adds its sample count (and an emission
+            // that was not positive) at the end. A slot sees the same sums.
+            const bool lit = any_positive(contribution);
+            if (lit) add_pixel(tg.radiance, pixel, contribution, 0.0f);
+
+            // The next-event candidate (a point on a light, or an
+            // environment direction) depends only on the hit point, so it is
+            // drawn before the closure is resolved; only the candidate stays
+            // live across the resolution.
+            const float envProb = s.env_select_prob;
+            const bool nee = bounce + 1 < depth_cap && light_sampling != CN_LIGHT_SAMPLING_BSDF &&
+                             (s.light_count > 0 || envProb > 0.0f);
+            LightConnection cand{S3(0.0f), 1e30f, S3(0.0f), 0.0f, false, LIGHT_NONE, 0u};
+            bool cand_env = false;
+            if (nee) {
+                const float uLight = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_LIGHT));
+                const f2 uPoint = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_POINT));
+                if (uLight < envProb) {
+                    const EnvSample e = env_sample(s, uPoint);
+                    cand.direction = e.direction;
+                    cand.radiance = e.radiance;
+                    cand.pdf = e.pdf;
+                    cand_env = true;
+                } else {
+                    const float uList = (uLight - envProb) / (1.0f - envProb);
+                    cand = sample_lights(s, uList, uPoint, spawn.position);
+                }
+            }
 
             const v3 shading = spawn.shading;
             const Frame frame = make_frame(shading);
@@ -428,52 +456,36 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             if (bounce + 1 >= depth_cap) {
                 terminal = true;
             } else {
-                // Next-event estimation (shade_surface.slang:363-446).
-                const float envProb = s.env_select_prob;
-                if (light_sampling != CN_LIGHT_SAMPLING_BSDF && (s.light_count > 0 || envProb > 0.0f)) {
-                    const float uLight = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_LIGHT));
-                    const f2 uPoint = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_POINT));
-                    v3 unshadowed = S3(0.0f);
-                    if (uLight < envProb) {
-                        const EnvSample cand = env_sample(s, uPoint);
-                        const v3 wiLight = to_local(frame, cand.direction);
-                        const bool through = dot(cand.direction, spawn.normal) < 0.0f;
-                        if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
-                            const float neePdf = envProb * cand.pdf;
-                            const float scatterPdf = bsdf_pdf(closure, wiLight, wo);
-                            const float weight =
-                                light_sampling == CN_LIGHT_SAMPLING_MIS ? power_heuristic(neePdf, scatterPdf) : 1.0f;
-                            unshadowed = (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
-                                         (weight / neePdf);
-                            sh_origin = through ? spawn.below : spawn.position;
-                            sh_light = LIGHT_NONE;
-                            sh_prim = 0;
-                            sh_dir = cand.direction;
-                            sh_tmax = 1e30f;
-                        }
-                    } else {
-                        const float uList = (uLight - envProb) / (1.0f - envProb);
-                        const LightConnection cand = sample_lights(s, uList, uPoint, spawn.position);
-                        const v3 wiLight = to_local(frame, cand.direction);
-                        const bool through = dot(cand.direction, spawn.normal) < 0.0f;
-                        if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
-                            const float neePdf = (1.0f - envProb) * cand.pdf;
-                            const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
-                                                     ? power_heuristic(neePdf, bsdf_pdf(closure, wiLight, wo))
-                                                     : 1.0f;
-                            unshadowed = (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
-                                         (weight / neePdf);
-                            sh_origin = through ? spawn.below : spawn.position;
-                            sh_light = cand.instance;
-                            sh_prim = cand.primitive;
-                            sh_dir = cand.direction;
-                            sh_tmax = cand.tmax;
+                // Next-event estimation (shade_surface.slang:363-446), with the
+                // candidate drawn above.
+                bool push_shadow = false;
+                v3 sh_origin = S3(0.0f), sh_contrib = S3(0.0f);
+                if (nee) {
+                    const v3 wiLight = to_local(frame, cand.direction);
+                    const bool through = dot(cand.direction, spawn.normal) < 0.0f;
+                    if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
+                        const float neePdf = (cand_env ? envProb : 1.0f - envProb) * cand.pdf;
+                        const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
+                                                 ? power_heuristic(neePdf, bsdf_pdf(closure, wiLight, wo))
+                                                 : 1.0f;
+                        const v3 unshadowed =
+                            (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
+                            (weight / neePdf);
+                        sh_origin = through ? spawn.below : spawn.position;
+                        if (any_positive(unshadowed)) {
+                            push_shadow = true;
+                            sh_contrib = unshadowed;
                         }
                     }
-                    if (any_positive(unshadowed)) {
-                        push_shadow = true;
-                        sh_contrib = unshadowed;
-                    }
+                }
+                // The connection goes out before scattering, so its record
+                // is dead through the rest of the shading.
+                const uint32_t ss = wave_claim(&counters[bounce * Q_COUNT + Q_SHADOW], push_shadow);
+                if (push_shadow) {
+                    p.shadow_origin[ss] = F4(sh_origin, cand.tmax);
+                    p.shadow_dir[ss] = F4(cand.direction, __uint_as_float(pixel));
+                    p.shadow_contrib[ss] = F4(sh_contrib, __uint_as_float(cand.instance));
+                    p.shadow_prim[ss] = cand.primitive;
                 }
 
                 // Scatter (shade_surface.slang:453-489).
@@ -515,7 +527,6 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                     p.throughput[path] = F4(scattered, scatter.pdf);
                     p.state[path] = (scatter.lobe << 24) | (medium & 0xffffffu);
                     push_ray = true;
-                    if (any_positive(contribution)) add_pixel(tg.radiance, pixel, contribution, 0.0f);
                 }
             }
             if (terminal) {
@@ -523,18 +534,11 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                     add_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f);
                     add_pixel(tg.normal, pixel, guide * shading, 0.0f);
                 }
-                add_pixel(tg.radiance, pixel, contribution, 1.0f);
+                add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
             }
         }
         const uint32_t rs = wave_claim(&counters[(bounce + 1) * Q_COUNT + Q_RAY], push_ray);
-        const uint32_t ss = wave_claim(&counters[bounce * Q_COUNT + Q_SHADOW], push_shadow);
         if (push_ray) next_rays[rs] = path;
-        if (push_shadow) {
-            p.shadow_origin[ss] = F4(sh_origin, sh_tmax);
-            p.shadow_dir[ss] = F4(sh_dir, __uint_as_float(pixel));
-            p.shadow_contrib[ss] = F4(sh_contrib, __uint_as_float(sh_light));
-            p.shadow_prim[ss] = sh_prim;
-        }
     }
 }
 

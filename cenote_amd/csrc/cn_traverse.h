@@ -259,6 +259,7 @@ struct TraceHit {
     float t;
     uint32_t inst, prim;
     float b1, b2;
+    uint32_t flags;  // the hit triangle's flags (mask, opacity, material class)
 };
 
 // Closest hit among triangles whose instance mask meets `mask`. Crossings
@@ -288,7 +289,7 @@ __device__ __forceinline__ void closest_leaf(const SceneView& s, const WRay& w, 
             ++crossings;
             if (!commit) continue;
         }
-        best = TraceHit{true, t, inst, prim, b1, b2};
+        best = TraceHit{true, t, inst, prim, b1, b2, flags};
     }
 }
 
@@ -297,7 +298,7 @@ __device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d
                                                   uint32_t& nodes_visited, uint32_t& tris_tested) {
     const SlabRay r = make_slab(o, d);
     const WRay w = make_wray(o, d);
-    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
+    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
     uint32_t crossings = 0;
     walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
         closest_leaf(s, w, ref, mask, pixel, sample, bounce, best, crossings, tris_tested);
@@ -327,7 +328,7 @@ __device__ __forceinline__ bool visibility_leaf(const SceneView& s, const WRay& 
             }
             if (t < t_early) return true;
         }
-        best = TraceHit{true, t, inst, prim, b1, b2};
+        best = TraceHit{true, t, inst, prim, b1, b2, 0u};
     }
     return false;
 }
@@ -350,7 +351,7 @@ __device__ __forceinline__ float trace_visibility(const SceneView& s, v3 o, v3 d
                                                   uint32_t& nodes_visited, uint32_t& tris_tested) {
     const SlabRay r = make_slab(o, d);
     const WRay w = make_wray(o, d);
-    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f};
+    TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
     bool occluded = false, translucent = false;
     walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
         occluded = visibility_leaf(s, w, ref, target_inst, target_prim, t_early, best, translucent, tris_tested);

@@ -71,6 +71,8 @@ enum {
  * need it), so that one wave carries 32 samples of a 1080p view and even
  * the deep-bounce launches fill the chip. */
 #define CN_POOL_CAPACITY (1u << 26)
+/* The largest pool a renderer may ask for. */
+#define CN_POOL_CAPACITY_MAX (1u << 27)
 
 /*
  * One instance's lowered OpenPBR surface, 144 bytes, field for field the
@@ -284,8 +286,9 @@ int cn_scene_env_tables(cn_scene* scene, float* marginal, float* conditional,
 int cn_scene_sample_environment(cn_scene* scene, const float* u, uint32_t count, float* out);
 
 /* Wavefront::new / Renderer::with_max_bounces (wavefront.rs:486-540,
- * render/mod.rs:151): capacity paths in the pool (0 = CN_POOL_CAPACITY),
- * max_bounces in 1..255, light_sampling a CN_LIGHT_SAMPLING_* value. */
+ * render/mod.rs:151): capacity paths in the pool (0 = CN_POOL_CAPACITY,
+ * at most CN_POOL_CAPACITY_MAX), max_bounces in 1..255, light_sampling a
+ * CN_LIGHT_SAMPLING_* value. */
 int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
                        uint32_t light_sampling, cn_renderer** out);
 int cn_renderer_destroy(cn_renderer* renderer);
