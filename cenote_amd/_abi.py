@@ -282,6 +282,7 @@ SIGNATURES = {
     "cn_scene_bvh": [_P, _P, C.c_uint32, _UP, _P, C.c_uint32, _UP],
     "cn_scene_light_records": [_P, C.POINTER(LightRecord), C.c_uint32, _UP],
     "cn_scene_env_tables": [_P, _FP, _FP, _FP],
+    "cn_bvh_quantize": [C.c_void_p, C.c_uint32, C.c_void_p],
     "cn_scene_sample_environment": [_P, _FP, C.c_uint32, _FP],
     "cn_renderer_create": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)],
     "cn_renderer_destroy": [_P],

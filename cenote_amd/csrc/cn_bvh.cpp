@@ -243,4 +243,61 @@ bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh
     return true;
 }
 
+static BvhNode4Q quantize_node(const BvhNode4& n) {
+    const float* lo[3] = {&n.lo_x.x, &n.lo_y.x, &n.lo_z.x};
+    const float* hi[3] = {&n.hi_x.x, &n.hi_y.x, &n.hi_z.x};
+    const uint32_t* child = &n.child.x;
+    BvhNode4Q q{};
+    float origin[3];
+    uint32_t planes_lo[3] = {0, 0, 0}, planes_hi[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        double a = INFINITY, b = -INFINITY;
+        for (int c = 0; c < 4; ++c) {
+            if (child[c] == CHILD_EMPTY) continue;
+            a = std::min(a, double(lo[k][c]));
+            b = std::max(b, double(hi[k][c]));
+        }
+        if (!(a <= b)) a = b = 0.0;  // no children: any grid
+        origin[k] = float(a);        // a is a float already: exact
+        // The smallest step 2^e (e >= -126, a normal float) with 255 steps
+        // covering [origin, b]; origins and planes are floats, so the
+        // differences below are exact in double.
+        const double extent = b - double(origin[k]);
+        int e = -126;
+        if (extent > 0.0) {
+            std::frexp(extent / 255.0, &e);
+            e = std::max(e, -126);
+            while (std::ldexp(255.0, e) < extent) ++e;
+            while (e > -126 && std::ldexp(255.0, e - 1) >= extent) --e;
+        }
+        const double step = std::ldexp(1.0, e);
+        for (int c = 0; c < 4; ++c) {
+            uint32_t ql = 0, qh = 0;
+            if (child[c] != CHILD_EMPTY) {
+                ql = uint32_t(std::floor((double(lo[k][c]) - double(origin[k])) / step));
+                qh = uint32_t(std::ceil((double(hi[k][c]) - double(origin[k])) / step));
+            }
+            planes_lo[k] |= std::min(ql, 255u) << (8 * c);
+            planes_hi[k] |= std::min(qh, 255u) << (8 * c);
+        }
+        q.exps |= uint32_t(e + 127) << (8 * k);
+    }
+    q.ox = origin[0];
+    q.oy = origin[1];
+    q.oz = origin[2];
+    q.lo_x = planes_lo[0];
+    q.hi_x = planes_hi[0];
+    q.lo_y = planes_lo[1];
+    q.hi_y = planes_hi[1];
+    q.lo_z = planes_lo[2];
+    q.hi_z = planes_hi[2];
+    q.depth = n.meta.x;
+    q.child = n.child;
+    return q;
+}
+
+void quantize_bvh4(const BvhNode4* nodes, size_t count, BvhNode4Q* out) {
+    for (size_t i = 0; i < count; ++i) out[i] = quantize_node(nodes[i]);
+}
+
 }  // namespace cn

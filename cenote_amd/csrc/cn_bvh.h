@@ -53,4 +53,10 @@ bool collapse_to_bvh4(const std::vector<BinaryNode>& bnodes, float max_abs, uint
 bool build_bvh4_device(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh4* out, BvhTriangle** tris,
                        std::string* error);
 
+// The traversal's 64-byte nodes (BvhNode4Q) from the full-precision ones,
+// node for node: each node's grid starts at the low corner of its
+// children's union and steps by the smallest power of two that spans the
+// union in 255 steps; planes round outward.
+void quantize_bvh4(const BvhNode4* nodes, size_t count, BvhNode4Q* out);
+
 }  // namespace cn

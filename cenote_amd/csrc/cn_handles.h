@@ -93,7 +93,8 @@ struct cn_scene {
     cn_context* ctx = nullptr;
     uint64_t serial = 0;  // unique per scene created in this process (graph-cache keys)
     cn_camera camera{};
-    cn::DevBuf<cn::BvhNode4> nodes;
+    cn::DevBuf<cn::BvhNode4Q> nodes;        // what the traversal reads
+    std::vector<cn::BvhNode4> host_nodes;    // the full-precision tree (cn_scene_bvh)
     cn::DevBuf<cn::BvhTriangle> tris;
     cn::DevBuf<cn::InstanceRecord> instances;
     cn::DevBuf<cn_material> materials;

@@ -237,7 +237,11 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     }
 
     // Upload.
-    CN_HIP(scene->nodes.upload(bvh.nodes.data(), bvh.nodes.size()));
+    {
+        std::vector<BvhNode4Q> quantized(bvh.nodes.size());
+        quantize_bvh4(bvh.nodes.data(), bvh.nodes.size(), quantized.data());
+        CN_HIP(scene->nodes.upload(quantized.data(), quantized.size()));
+    }
     CN_HIP(scene->instances.upload(records.data(), records.size()));
     CN_HIP(scene->materials.upload(materials.data(), materials.size()));
     if (scene->host_lights.empty()) {
@@ -291,8 +295,9 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out) 
     info.env_select_prob = select;
     info.env_power = scene->host_env.power;
     info.bvh_node_count = uint32_t(bvh.nodes.size());
+    scene->host_nodes = std::move(bvh.nodes);
     info.bvh_width = 4;
-    info.bvh_node_bytes = sizeof(BvhNode4);
+    info.bvh_node_bytes = sizeof(BvhNode4Q);
     info.bvh_triangle_bytes = sizeof(BvhTriangle);
     info.bvh_max_depth = bvh.max_depth;
     info.bvh_builder = desc->bvh_builder;
@@ -313,16 +318,22 @@ int cn_scene_bvh(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t*
                  uint32_t triangle_capacity, uint32_t* triangle_count) {
     if (!scene) return fail(CN_E_INVALID, "cn_scene_bvh: null scene");
     CN_HIP(hipSetDevice(scene->ctx->device));
-    if (node_count) *node_count = uint32_t(scene->nodes.count);
+    if (node_count) *node_count = uint32_t(scene->host_nodes.size());
     if (triangle_count) *triangle_count = uint32_t(scene->tris.count);
     if (nodes) {
-        if (node_capacity < scene->nodes.count) return fail(CN_E_INVALID, "node buffer too small");
-        CN_HIP(hipMemcpy(nodes, scene->nodes.ptr, scene->nodes.count * sizeof(BvhNode4), hipMemcpyDeviceToHost));
+        if (node_capacity < scene->host_nodes.size()) return fail(CN_E_INVALID, "node buffer too small");
+        std::memcpy(nodes, scene->host_nodes.data(), scene->host_nodes.size() * sizeof(BvhNode4));
     }
     if (triangles) {
         if (triangle_capacity < scene->tris.count) return fail(CN_E_INVALID, "triangle buffer too small");
         CN_HIP(hipMemcpy(triangles, scene->tris.ptr, scene->tris.count * sizeof(BvhTriangle), hipMemcpyDeviceToHost));
     }
+    return CN_OK;
+}
+
+int cn_bvh_quantize(const void* nodes, uint32_t count, void* out) {
+    if (count && (!nodes || !out)) return fail(CN_E_INVALID, "cn_bvh_quantize: null argument");
+    quantize_bvh4(static_cast<const BvhNode4*>(nodes), count, static_cast<BvhNode4Q*>(out));
     return CN_OK;
 }
 

@@ -153,8 +153,13 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
 // Two half-empty launches per bounce become one fuller one, which is what
 // the deep, sparse bounces need.
 
-__global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
-                                                           uint32_t range_count, uint32_t bounce, int counting) {
+// COUNT: accumulate node and triangle fetches (the counting pass); the
+// timed form keeps no counters. Eight waves per SIMD (64 VGPRs; LDS 8 x 16
+// KB per CU) fit without scratch: traversal waits on memory, so every
+// extra wave hides latency.
+template <bool COUNT>
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                           uint32_t range_count, uint32_t bounce) {
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
     const uint32_t n_ray = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
     const uint32_t n_shadow = bounce == 0 ? 0u : counters[(bounce - 1) * Q_COUNT + Q_SHADOW];
@@ -201,7 +206,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_trace(SceneView s, PathPool
             }
         }
     }
-    if (counting) {
+    if constexpr (COUNT) {
         wave_add_u64(&p.traversal_counts[0], ray_nodes);
         wave_add_u64(&p.traversal_counts[1], ray_tris);
         wave_add_u64(&p.traversal_counts[2], shadow_nodes);
@@ -734,8 +739,12 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     // Bounce b's rays and bounce b-1's shadow rays share one launch; the
     // last bounce shades terminally and leaves no shadow rays behind.
     if (prof) prof->begin(K_TRACE);
-    hipLaunchKernelGGL(k_trace, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count, bounce,
-                       counting);
+    if (counting)
+        hipLaunchKernelGGL(k_trace<true>, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count,
+                           bounce);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count,
+                           bounce);
     if (prof) prof->end(K_TRACE);
     if (prof) prof->begin(K_SHADE_MISS);
     hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,

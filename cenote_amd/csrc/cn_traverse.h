@@ -152,7 +152,7 @@ __device__ __forceinline__ bool hit_triangle(const WRay& w, v3 v0, v3 v1, v3 v2,
 }
 
 struct SlabRay {
-    float ix, iy, iz, ox, oy, oz;
+    float ix, iy, iz, x, y, z;  // reciprocal direction, origin
 };
 
 __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
@@ -160,11 +160,14 @@ __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
     r.ix = safe_rcp(d.x);
     r.iy = safe_rcp(d.y);
     r.iz = safe_rcp(d.z);
-    r.ox = o.x * r.ix;
-    r.oy = o.y * r.iy;
-    r.oz = o.z * r.iz;
+    r.x = o.x;
+    r.y = o.y;
+    r.z = o.z;
     return r;
 }
+
+// Byte c of a plane word as a float (one v_cvt_f32_ubyte).
+__device__ __forceinline__ float plane(uint32_t word, int c) { return float((word >> (8 * c)) & 0xffu); }
 
 // One compare-exchange of a stable sort by key: adjacent positions only,
 // swapping on strictly greater, so equal keys keep their slot order.
@@ -183,24 +186,25 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 // Hit children are ordered by entry distance, ties in slot order: an
 // odd-even transposition network (six adjacent compare-exchanges, no
 // branches), misses keyed +inf so they sort last.
-__device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ nodes, uint32_t index, const SlabRay& r,
+// A plane at origin + q * step meets the ray at t = q * (step / d) +
+// (origin - o) / d: one fma per plane after six per-node products. The
+// boxes' padding (far above these roundings) keeps the test conservative.
+__device__ __forceinline__ uint32_t visit_node(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                                float bound, Stack& st) {
-    const BvhNode4 n = nodes[index];
-    const float lx[4] = {n.lo_x.x, n.lo_x.y, n.lo_x.z, n.lo_x.w};
-    const float hx[4] = {n.hi_x.x, n.hi_x.y, n.hi_x.z, n.hi_x.w};
-    const float ly[4] = {n.lo_y.x, n.lo_y.y, n.lo_y.z, n.lo_y.w};
-    const float hy[4] = {n.hi_y.x, n.hi_y.y, n.hi_y.z, n.hi_y.w};
-    const float lz[4] = {n.lo_z.x, n.lo_z.y, n.lo_z.z, n.lo_z.w};
-    const float hz[4] = {n.hi_z.x, n.hi_z.y, n.hi_z.z, n.hi_z.w};
+    const BvhNode4Q n = nodes[index];
+    const float sx = __uint_as_float((n.exps & 0xffu) << 23) * r.ix;
+    const float sy = __uint_as_float(((n.exps >> 8) & 0xffu) << 23) * r.iy;
+    const float sz = __uint_as_float(((n.exps >> 16) & 0xffu) << 23) * r.iz;
+    const float bx0 = (n.ox - r.x) * r.ix, by0 = (n.oy - r.y) * r.iy, bz0 = (n.oz - r.z) * r.iz;
     const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
     float key[4];
     uint32_t ref[4];
     uint32_t nh = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float ax = fmaf(lx[c], r.ix, -r.ox), bx = fmaf(hx[c], r.ix, -r.ox);
-        const float ay = fmaf(ly[c], r.iy, -r.oy), by = fmaf(hy[c], r.iy, -r.oy);
-        const float az = fmaf(lz[c], r.iz, -r.oz), bz = fmaf(hz[c], r.iz, -r.oz);
+        const float ax = fmaf(plane(n.lo_x, c), sx, bx0), bx = fmaf(plane(n.hi_x, c), sx, bx0);
+        const float ay = fmaf(plane(n.lo_y, c), sy, by0), by = fmaf(plane(n.hi_y, c), sy, by0);
+        const float az = fmaf(plane(n.lo_z, c), sz, bz0), bz = fmaf(plane(n.hi_z, c), sz, bz0);
         const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
         const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound)) * 1.0000004f;
         const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
@@ -222,7 +226,7 @@ __device__ __forceinline__ uint32_t visit_node(const BvhNode4* __restrict__ node
 // The speculative while-while walk over [0, bound]. `leaf(ref)` tests one
 // leaf's triangles (it may shrink `bound`) and returns true to end the ray.
 template <typename Leaf>
-__device__ __forceinline__ void walk(const BvhNode4* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
+__device__ __forceinline__ void walk(const BvhNode4Q* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
                                      uint32_t& nodes_visited, Leaf&& leaf_fn) {
     st.sp = 0;
     uint32_t cur = 0;  // the root is always an internal node

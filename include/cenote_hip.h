@@ -269,10 +269,18 @@ int cn_scene_destroy(cn_scene* scene);
 /* Camera edits between frames (Scene::camera_mut, scene/mod.rs:391-394). */
 int cn_scene_set_camera(cn_scene* scene, const cn_camera* camera);
 int cn_scene_info_get(cn_scene* scene, cn_scene_info* info);
-/* The resident BVH as raw records: 128-byte nodes and 48-byte leaf-order
- * triangles (either output may be NULL; *count gets the record count). */
+/* The BVH as raw records: the full-precision 128-byte nodes (six float4 of
+ * child slab bounds, child refs, meta) and the 48-byte leaf-order triangles
+ * the traversal reads (either output may be NULL; *count gets the record
+ * count). The traversal reads each node quantized to 64 bytes, as
+ * cn_bvh_quantize computes it. */
 int cn_scene_bvh(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t* node_count, void* triangles,
                  uint32_t triangle_capacity, uint32_t* triangle_count);
+/* The 64-byte traversal nodes for `count` 128-byte nodes (host memory;
+ * no device is involved): per node, an origin, three power-of-two grid
+ * steps and each child's planes as bytes rounded outward, so every
+ * quantized child box contains its full-precision box. */
+int cn_bvh_quantize(const void* nodes, uint32_t count, void* out);
 /* Host copies of the bit-exact tables, for parity checks. */
 int cn_scene_light_records(cn_scene* scene, cn_light_record* out, uint32_t capacity,
                            uint32_t* count);

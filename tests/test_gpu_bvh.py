@@ -12,6 +12,8 @@ import cenote_amd as cn
 from cenote_amd import _abi, workloads
 from cenote_amd.scene import Material, MeshData, SceneData, identity
 
+from bvh_nodes import contains, quantize
+
 pytestmark = pytest.mark.gpu
 
 
@@ -78,3 +80,11 @@ def test_heavy_scene_builds_the_same_on_the_device(ctx):
     assert int(np.count_nonzero(dn != hn)) == 0
     assert int(np.count_nonzero(dt != ht)) == 0
     print(f"heavy BVH build: device {di.bvh_build_seconds:.3f} s, host {hi.bvh_build_seconds:.3f} s")
+
+
+@pytest.mark.parametrize("name", ["chart", "soup_wide_100k", "degenerate_signed_zeros"])
+def test_traversal_nodes_contain_the_tree(ctx, name):
+    """The 64-byte nodes the traversal reads bound the full-precision tree."""
+    nodes, _, info = built(ctx, CASES[name](), _abi.BVH_HOST)
+    assert info.bvh_node_bytes == 64
+    assert contains(nodes, quantize(nodes))
