@@ -204,3 +204,76 @@ def loads(text: str):
     if p.peek()[0] != "end":
         p.fail("end of file")
     return v
+
+
+def _dump_str(text: str) -> str:
+    out = []
+    for ch in text:
+        if ch in ('"', "\\"):
+            out.append("\\" + ch)
+        elif ch == "\n":
+            out.append("\\n")
+        elif ch == "\t":
+            out.append("\\t")
+        elif ch == "\r":
+            out.append("\\r")
+        elif ch == "\0":
+            out.append("\\0")
+        else:
+            out.append(ch)
+    return '"' + "".join(out) + '"'
+
+
+def _dump_float(x: float) -> str:
+    if x != x:
+        return "NaN"
+    if x in (float("inf"), float("-inf")):
+        return "inf" if x > 0 else "-inf"
+    text = repr(float(x))
+    return text if any(c in text for c in ".eE") else text + ".0"
+
+
+def dumps(value, indent: int = 0) -> str:
+    """One RON value as text that ``loads`` reads back to an equal value
+    (structs as named-field groups, variants with their payload, floats in
+    their shortest round-tripping form)."""
+    pad, inner = "    " * indent, "    " * (indent + 1)
+    if value is None:
+        return "None"
+    if value is True:
+        return "true"
+    if value is False:
+        return "false"
+    if isinstance(value, int):
+        return str(value)
+    if isinstance(value, float):
+        return _dump_float(value)
+    if isinstance(value, str):
+        return _dump_str(value)
+    if isinstance(value, Some):
+        return f"Some({dumps(value.value, indent)})"
+    if isinstance(value, Variant):
+        if value.value is None:
+            return value.name
+        if isinstance(value.value, dict):
+            return value.name + dumps(value.value, indent)
+        if isinstance(value.value, tuple):
+            return value.name + "(" + ", ".join(dumps(v, indent) for v in value.value) + ")"
+        return f"{value.name}({dumps(value.value, indent)})"
+    if isinstance(value, dict):
+        if not value:
+            return "()"
+        fields = [f"{inner}{k}: {dumps(v, indent + 1)}," for k, v in value.items()]
+        return "(\n" + "\n".join(fields) + "\n" + pad + ")"
+    if isinstance(value, tuple):
+        if len(value) == 1:
+            return f"({dumps(value[0], indent)},)"
+        return "(" + ", ".join(dumps(v, indent) for v in value) + ")"
+    if isinstance(value, list):
+        if not value:
+            return "[]"
+        if all(not isinstance(v, (dict, Variant, Some)) for v in value):
+            return "[" + ", ".join(dumps(v, indent) for v in value) + "]"
+        return "[\n" + "\n".join(f"{inner}{dumps(v, indent + 1)}," for v in value) + "\n" + pad + "]"
+    raise RonError(f"cannot write {type(value).__name__} as RON")
+

@@ -122,6 +122,12 @@ def _check_op(op):
         raise SceneFileError("scene file parse failed: name must be a string")
 
 
+def paths_of(op):
+    """The filesystem paths an op references, as (container, key) pairs
+    (the containers are the op's own dicts: assigning rewrites the op)."""
+    return _paths(op)
+
+
 def _paths(op):
     """The filesystem paths an op references, as (container, key) pairs."""
     out = []
