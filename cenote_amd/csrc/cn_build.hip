@@ -627,7 +627,7 @@ bool build_bvh4_device(const std::vector<WorldTriangle>& input, uint32_t max_sta
                     sp.scale = bvh_scale(cbox.lo[d.axis], cbox.hi[d.axis]);
                 }
             }
-            // Ranges this large always split (a leaf needs <= 4 triangles).
+            // Ranges this large always split (a leaf holds at most BVH_LEAF_SIZE triangles).
             BinaryNode& node = top[rg.node];
             node.box = box;
             node.left = next_node;

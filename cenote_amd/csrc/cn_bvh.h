@@ -34,7 +34,7 @@ struct BinaryNode {
 };
 static_assert(sizeof(BinaryNode) == 40, "device and host share this layout");
 
-// Binned-SAH binary build (32 bins on the three axes, leaves of <= 4
+// Binned-SAH binary build (32 bins on the three axes, leaves of <= 8
 // triangles), collapsed to 4-wide nodes by repeatedly opening the child of
 // largest surface area. Returns false with a message when the tree would
 // be deeper than the traversal stack allows.

@@ -28,9 +28,11 @@ from .scene import (CameraData, Material, MeshData, OpenPBR, SceneData, constant
 from .shapes import geodesic_sphere, ground_plane, uv_sphere, vertex_normals
 
 # The workload shapes the benchmark and tests use (pixels, spp, depth).
+# Every film accumulation writes the albedo, normal and depth guides with the
+# beauty (render/film.rs:66-127), so every configuration carries AOVs.
 CONFIGS = {
-    "cornell": dict(width=256, height=256, spp=16, max_bounces=8, aov=False),
-    "chart": dict(width=1280, height=720, spp=64, max_bounces=12, aov=False),
+    "cornell": dict(width=256, height=256, spp=16, max_bounces=8, aov=True),
+    "chart": dict(width=1280, height=720, spp=64, max_bounces=12, aov=True),
     "heavy": dict(width=1920, height=1080, spp=32, max_bounces=16, aov=True),
 }
 
