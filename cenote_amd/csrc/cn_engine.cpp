@@ -81,15 +81,20 @@ int ensure_pool(cn_renderer* r, size_t paths) {
     CN_HIP(r->hit.alloc(paths));
     CN_HIP(r->key.alloc(paths));
     CN_HIP(r->state.alloc(paths));
-    CN_HIP(r->ray_queue0.alloc(paths));
-    CN_HIP(r->ray_queue1.alloc(paths));
-    CN_HIP(r->hit_queue.alloc(paths));
-    CN_HIP(r->miss_queue.alloc(paths));
-    CN_HIP(r->shadow_origin.alloc(paths));
-    CN_HIP(r->shadow_dir.alloc(paths));
-    CN_HIP(r->shadow_contrib.alloc(paths));
-    CN_HIP(r->shadow_prim.alloc(paths));
+    // Every queue (and the shadow records) holds QUEUE_SPLIT sub-queues, each
+    // sized for the most entries a pool-sized input can send it.
+    const uint32_t sub = sub_capacity_for(uint32_t(paths));
+    const size_t entries = size_t(sub) * QUEUE_SPLIT;
+    CN_HIP(r->ray_queue0.alloc(entries));
+    CN_HIP(r->ray_queue1.alloc(entries));
+    CN_HIP(r->hit_queue.alloc(entries));
+    CN_HIP(r->miss_queue.alloc(entries));
+    CN_HIP(r->shadow_origin.alloc(entries));
+    CN_HIP(r->shadow_dir.alloc(entries));
+    CN_HIP(r->shadow_contrib.alloc(entries));
+    CN_HIP(r->shadow_prim.alloc(entries));
     PathPool& p = r->pool;
+    p.sub_capacity = sub;
     p.origin = r->origin.ptr;
     p.direction = r->direction.ptr;
     p.hit = r->hit.ptr;
@@ -237,8 +242,8 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
             // counter sets are reused past MAX_RANGES (only the per-wave
             // diagnostics need them distinct).
             uint32_t* counters = r->counters.ptr + (k % MAX_RANGES) * per;
-            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), r->pool, tg, counters, per, rg.base,
-                                 rg.count, r->profiler.get()));
+            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), r->pool, tg, counters,
+                                 counter_lines_per_range(r->max_bounces), rg.base, rg.count, r->profiler.get()));
             for (uint32_t b = 0; b < rg.rounds; ++b) {
                 CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, r->pool, tg, counters, rg.base,
                                      rg.count, b, r->light_sampling, r->counting, r->profiler.get()));
@@ -494,19 +499,23 @@ int cn_renderer_wave_counts(cn_renderer* r, cn_wave_counts* out) {
         return fail(CN_E_INVALID, "the last wave walked %u pool ranges; counts cover at most %u", r->last_ranges,
                     MAX_RANGES);
     const uint32_t per = counters_per_range(r->max_bounces);
-    std::vector<uint32_t> c(size_t(per) * MAX_RANGES);
+    std::vector<uint32_t> c(size_t(per) * r->last_ranges);
     CN_HIP(hipMemcpy(c.data(), r->counters.ptr, c.size() * 4, hipMemcpyDeviceToHost));
     std::memset(out, 0, sizeof(*out));
     out->bounces = r->max_bounces;
     for (uint32_t range = 0; range < r->last_ranges; ++range) {
         const uint32_t* rc = &c[size_t(range) * per];
+        auto count = [&](uint32_t b, Queue q) {
+            uint32_t sum = 0;
+            for (uint32_t k = 0; k < QUEUE_SPLIT; ++k) sum += rc[counter_index(b, q, k)];
+            return sum;
+        };
         for (uint32_t b = 0; b < r->max_bounces && b < 256; ++b) {
-            out->rays[b] += b == 0 ? 0 : rc[b * Q_COUNT + Q_RAY];
-            out->hits[b] += rc[b * Q_COUNT + Q_HIT];
-            out->misses[b] += rc[b * Q_COUNT + Q_MISS];
-            out->shadows[b] += rc[b * Q_COUNT + Q_SHADOW];
+            out->rays[b] += b == 0 ? count(0, Q_HIT) + count(0, Q_MISS) : count(b, Q_RAY);
+            out->hits[b] += count(b, Q_HIT);
+            out->misses[b] += count(b, Q_MISS);
+            out->shadows[b] += count(b, Q_SHADOW);
         }
-        out->rays[0] += rc[Q_HIT] + rc[Q_MISS];
     }
     return CN_OK;
 }

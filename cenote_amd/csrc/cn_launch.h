@@ -62,7 +62,7 @@ struct Profiler {
 // Camera paths for slots [base, base + count) of a wave whose jobs (sorted
 // by slot base) are `jobs[0 .. job_count)`.
 hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_count, uint32_t base,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_lines, uint32_t base,
                          uint32_t count, Profiler* prof);
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,

@@ -8,8 +8,9 @@
 // grid and walks its queue with a grid-stride loop, reading the count the
 // previous kernel left (HIP's stand-in for the reference's indirect
 // dispatch, wavefront.rs:799-828). Queue pushes are aggregated per wave64 —
-// one ballot, one popcount, one atomic per wave — instead of the one global
-// atomic per push of shaders/pathstate.slang:277-283.
+// one ballot, one popcount, one atomic per wave, on one of QUEUE_SPLIT
+// counters per queue — instead of the one global atomic per push of
+// shaders/pathstate.slang:277-283.
 //
 // Radiance and AOV writes are pixel-owned read-modify-writes: a wave holds
 // one path per pixel, and stream order serialises the stages, so the adds
@@ -44,6 +45,37 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* counter, bool active) {
     if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     return base + below;
+}
+
+// The sub-queue a producer lane claims in: that of its wave's 64-item input
+// chunk. `first` is the block's first input item (a multiple of 64).
+__device__ __forceinline__ uint32_t chunk_split(uint32_t first) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / QUEUE_CHUNK);  // wave-uniform
+    return (first / QUEUE_CHUNK + wave) % QUEUE_SPLIT;
+}
+
+// A consumer sees a queue as its sub-queues laid end to end. The counts are
+// reread (scalar loads, cached) wherever they are needed rather than kept
+// live through a kernel's loop, which cost registers.
+__device__ __forceinline__ uint32_t queue_total(const uint32_t* counters, uint32_t bounce, Queue queue) {
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < QUEUE_SPLIT; ++s) total += counters[counter_index(bounce, queue, s)];
+    return total;
+}
+
+// Array position of the queue's i-th entry (i < its total).
+__device__ __forceinline__ uint32_t queue_slot(const uint32_t* counters, uint32_t bounce, Queue queue, uint32_t i,
+                                               uint32_t sub_capacity) {
+    uint32_t at = i, base = 0, start = 0;
+#pragma unroll
+    for (uint32_t s = 0; s + 1 < QUEUE_SPLIT; ++s) {
+        start += counters[counter_index(bounce, queue, s)];
+        const bool past = i >= start;
+        at = past ? i - start : at;
+        base = past ? (s + 1) * sub_capacity : base;
+    }
+    return base + at;
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* target, uint32_t value) {
@@ -108,13 +140,13 @@ __device__ __forceinline__ void camera_ray(const CameraView& cam, uint32_t width
 // One thread per slot of the range: find the slot's job (jobs sorted by
 // slot base), derive its pixel and sample, and start the camera path. Also
 // zero-fills the slot's accumulators (the reference's Pass::Fill,
-// wavefront.rs:733-759) and resets the range's queue counters: raygen owns
-// every slot exactly once.
+// wavefront.rs:733-759) and resets the range's queue counters (one word per
+// 128-byte counter line): raygen owns every slot exactly once.
 __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ jobs, uint32_t job_count, PathPool p,
-                                                  WaveTargets tg, uint32_t* counters, uint32_t counter_count,
+                                                  WaveTargets tg, uint32_t* counters, uint32_t counter_lines,
                                                   uint32_t base, uint32_t count) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < counter_count) counters[i] = 0;
+    if (i < counter_lines) counters[i * COUNTER_STRIDE] = 0;
     if (i >= count) return;
     const uint32_t slot = base + i;
     uint32_t lo = 0, hi = job_count;
@@ -161,9 +193,9 @@ template <bool COUNT>
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                            uint32_t range_count, uint32_t bounce) {
     __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
-    const uint32_t n_ray = bounce == 0 ? range_count : counters[bounce * Q_COUNT + Q_RAY];
-    const uint32_t n_shadow = bounce == 0 ? 0u : counters[(bounce - 1) * Q_COUNT + Q_SHADOW];
-    const uint32_t n = n_ray + n_shadow;
+    // Bounce 0 traces the camera paths: item i is path i.
+    const uint32_t n_ray = bounce == 0 ? range_count : queue_total(counters, bounce, Q_RAY);
+    const uint32_t n = n_ray + (bounce == 0 ? 0u : queue_total(counters, bounce - 1, Q_SHADOW));
     const uint32_t* rays = p.ray_queue[bounce & 1];
     const uint32_t lanes = gridDim.x * BLOCK;
     Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x);
@@ -175,7 +207,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         uint32_t path = 0, cls = 0;
         bool hit = false;
         if (is_ray) {
-            path = bounce == 0 ? i : rays[i];
+            path = bounce == 0 ? i : rays[queue_slot(counters, bounce, Q_RAY, i, p.sub_capacity)];
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
             const uint2 key = p.key[path];
@@ -184,7 +216,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
             cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
         } else if (i < n) {
-            const uint32_t j = i - n_ray;
+            const uint32_t j = queue_slot(counters, bounce - 1, Q_SHADOW, i - n_ray, p.sub_capacity);
             const float4 o4 = p.shadow_origin[j];
             const float4 d4 = p.shadow_dir[j];
             const float4 c4 = p.shadow_contrib[j];
@@ -198,11 +230,13 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             if (att >= 1e-4f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
         }
         if (first < n_ray) {  // uniform per block: some wave of this block holds rays
-            const uint32_t hs = wave_claim(&counters[bounce * Q_COUNT + Q_HIT], is_ray && hit);
-            const uint32_t ms = wave_claim(&counters[bounce * Q_COUNT + Q_MISS], is_ray && !hit);
+            const uint32_t split = chunk_split(first);
+            const uint32_t hs = wave_claim(&counters[counter_index(bounce, Q_HIT, split)], is_ray && hit);
+            const uint32_t ms = wave_claim(&counters[counter_index(bounce, Q_MISS, split)], is_ray && !hit);
             if (is_ray) {
-                if (hit) p.hit_queue[hs] = path | (cls << HIT_PATH_BITS);
-                else p.miss_queue[ms] = path;
+                const uint32_t at = split * p.sub_capacity;
+                if (hit) p.hit_queue[at + hs] = path | (cls << HIT_PATH_BITS);
+                else p.miss_queue[at + ms] = path;
             }
         }
     }
@@ -218,9 +252,9 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
 
 __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
-    const uint32_t n = counters[bounce * Q_COUNT + Q_MISS];
+    const uint32_t n = queue_total(counters, bounce, Q_MISS);
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        const uint32_t path = p.miss_queue[i];
+        const uint32_t path = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
         const uint32_t pixel = base + path;
         const v3 direction = xyz(p.direction[path]);
         const float4 thr = p.throughput[path];
@@ -326,7 +360,7 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
 template <uint32_t SB, bool LDS_TABLES, bool SORT>
 __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
-    const uint32_t n = counters[bounce * Q_COUNT + Q_HIT];
+    const uint32_t n = queue_total(counters, bounce, Q_HIT);
     if (blockIdx.x * SB >= n) return;  // uniform per block: nothing to shade here
     const float* tables = s.tables;
     if constexpr (LDS_TABLES) {
@@ -343,7 +377,7 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
     for (uint32_t first = blockIdx.x * SB; first < n; first += gridDim.x * SB) {
         const uint32_t i = first + threadIdx.x;
         bool valid = i < n;
-        const uint32_t entry = valid ? p.hit_queue[i] : 0u;
+        const uint32_t entry = valid ? p.hit_queue[queue_slot(counters, bounce, Q_HIT, i, p.sub_capacity)] : 0u;
         uint32_t path = entry & HIT_PATH_MASK;
         if constexpr (SORT) {
             // Bucket MATERIAL_CLASSES holds the lanes past the queue's end.
@@ -485,7 +519,9 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                 }
                 // The connection goes out before scattering, so its record
                 // is dead through the rest of the shading.
-                const uint32_t ss = wave_claim(&counters[bounce * Q_COUNT + Q_SHADOW], push_shadow);
+                const uint32_t split = chunk_split(first);
+                const uint32_t ss = split * p.sub_capacity +
+                                    wave_claim(&counters[counter_index(bounce, Q_SHADOW, split)], push_shadow);
                 if (push_shadow) {
                     p.shadow_origin[ss] = F4(sh_origin, cand.tmax);
                     p.shadow_dir[ss] = F4(cand.direction, __uint_as_float(pixel));
@@ -542,8 +578,9 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                 add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
             }
         }
-        const uint32_t rs = wave_claim(&counters[(bounce + 1) * Q_COUNT + Q_RAY], push_ray);
-        if (push_ray) next_rays[rs] = path;
+        const uint32_t split = chunk_split(first);
+        const uint32_t rs = wave_claim(&counters[counter_index(bounce + 1, Q_RAY, split)], push_ray);
+        if (push_ray) next_rays[split * p.sub_capacity + rs] = path;
     }
 }
 
@@ -720,12 +757,12 @@ static uint32_t grid_for(uint32_t items, uint32_t cap) {
 }
 
 hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_count, uint32_t base,
+                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_lines, uint32_t base,
                          uint32_t count, Profiler* prof) {
-    const uint32_t items = count > counter_count ? count : counter_count;
+    const uint32_t items = count > counter_lines ? count : counter_lines;
     if (prof) prof->begin(K_RAYGEN);
     hipLaunchKernelGGL(k_raygen, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, jobs, job_count, p, tg,
-                       counters, counter_count, base, count);
+                       counters, counter_lines, base, count);
     if (prof) prof->end(K_RAYGEN);
     return hipGetLastError();
 }
