@@ -24,8 +24,11 @@ enum Queue : uint32_t { Q_RAY = 0, Q_HIT = 1, Q_MISS = 2, Q_SHADOW = 3, Q_COUNT 
 // the trace kernel's time. A producer wave claims in the sub-queue of its
 // 64-item input chunk (chunk index mod QUEUE_SPLIT); sub-queue s keeps its
 // entries at [s * sub_capacity, s * sub_capacity + count_s) of the queue's
-// array, and a consumer walks the sub-queues one after another.
-constexpr uint32_t QUEUE_SPLIT = 16;
+// array, and a consumer walks the sub-queues one after another. Measured on
+// the chart and 2M-triangle scenes: 4 and 8 sub-queues alike, 16 to 64
+// slower (each consumer item walks the counts); 8 keeps twice the claim
+// headroom of 4.
+constexpr uint32_t QUEUE_SPLIT = 8;
 constexpr uint32_t QUEUE_CHUNK = 64;      // input items per claiming wave
 constexpr uint32_t COUNTER_STRIDE = 32;   // u32 words per counter: one 128-byte line
 

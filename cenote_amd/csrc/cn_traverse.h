@@ -153,6 +153,7 @@ __device__ __forceinline__ bool hit_triangle(const WRay& w, v3 v0, v3 v1, v3 v2,
 
 struct SlabRay {
     float ix, iy, iz, x, y, z;  // reciprocal direction, origin
+    bool nx, ny, nz;            // the axis's reciprocal is negative: the near plane is the high one
 };
 
 __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
@@ -163,6 +164,9 @@ __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
     r.x = o.x;
     r.y = o.y;
     r.z = o.z;
+    r.nx = r.ix < 0.0f;
+    r.ny = r.iy < 0.0f;
+    r.nz = r.iz < 0.0f;
     return r;
 }
 
@@ -188,7 +192,12 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 // branches), misses keyed +inf so they sort last.
 // A plane at origin + q * step meets the ray at t = q * (step / d) +
 // (origin - o) / d: one fma per plane after six per-node products. The
-// boxes' padding (far above these roundings) keeps the test conservative.
+// step is positive, so on each axis the low plane is met first exactly
+// when d is positive: the near and far plane words are picked once per node
+// by the ray's signs, and the per-child test needs no min/max per axis
+// (the same values the min/max form gives; a child's low byte never
+// exceeds its high one). The boxes' padding (far above these roundings)
+// keeps the test conservative.
 __device__ __forceinline__ uint32_t visit_node(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                                float bound, Stack& st) {
     const BvhNode4Q n = nodes[index];
@@ -196,17 +205,20 @@ __device__ __forceinline__ uint32_t visit_node(const BvhNode4Q* __restrict__ nod
     const float sy = __uint_as_float(((n.exps >> 8) & 0xffu) << 23) * r.iy;
     const float sz = __uint_as_float(((n.exps >> 16) & 0xffu) << 23) * r.iz;
     const float bx0 = (n.ox - r.x) * r.ix, by0 = (n.oy - r.y) * r.iy, bz0 = (n.oz - r.z) * r.iz;
+    const uint32_t near_x = r.nx ? n.hi_x : n.lo_x, far_x = r.nx ? n.lo_x : n.hi_x;
+    const uint32_t near_y = r.ny ? n.hi_y : n.lo_y, far_y = r.ny ? n.lo_y : n.hi_y;
+    const uint32_t near_z = r.nz ? n.hi_z : n.lo_z, far_z = r.nz ? n.lo_z : n.hi_z;
     const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
     float key[4];
     uint32_t ref[4];
     uint32_t nh = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float ax = fmaf(plane(n.lo_x, c), sx, bx0), bx = fmaf(plane(n.hi_x, c), sx, bx0);
-        const float ay = fmaf(plane(n.lo_y, c), sy, by0), by = fmaf(plane(n.hi_y, c), sy, by0);
-        const float az = fmaf(plane(n.lo_z, c), sz, bz0), bz = fmaf(plane(n.hi_z, c), sz, bz0);
-        const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-        const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound)) * 1.0000004f;
+        const float ax = fmaf(plane(near_x, c), sx, bx0), bx = fmaf(plane(far_x, c), sx, bx0);
+        const float ay = fmaf(plane(near_y, c), sy, by0), by = fmaf(plane(far_y, c), sy, by0);
+        const float az = fmaf(plane(near_z, c), sz, bz0), bz = fmaf(plane(far_z, c), sz, bz0);
+        const float t0 = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+        const float t1 = fminf(fminf(bx, by), fminf(bz, bound)) * 1.0000004f;
         const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
         key[c] = hit ? t0 : __int_as_float(0x7f800000);
         ref[c] = hit ? ch[c] : NODE_DONE;
