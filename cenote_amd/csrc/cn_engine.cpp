@@ -352,7 +352,7 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     r->shape.shade_blocks = cus * 8;
     r->shape.light_blocks = cus * 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-    CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * 256 * STACK_SPILL));
+    CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * TRAVERSAL_BLOCK * STACK_SPILL));
     CN_HIP(r->traversal.alloc(4));
     CN_HIP(hipMemset(r->traversal.ptr, 0, 4 * sizeof(unsigned long long)));
     CN_HIP(r->counters.alloc(size_t(MAX_RANGES) * counters_per_range(max_bounces)));
@@ -408,7 +408,7 @@ int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
     // The traversal stack's spill area is per lane of the trace grid.
-    const size_t spill = size_t(shape->trace_blocks) * 256 * STACK_SPILL;
+    const size_t spill = size_t(shape->trace_blocks) * TRAVERSAL_BLOCK * STACK_SPILL;
     if (r->spill.count < spill) {
         r->drop_graphs();
         CN_HIP(r->spill.alloc(spill));

@@ -14,6 +14,8 @@ namespace cn {
 constexpr uint32_t STACK_LDS = 16;
 constexpr uint32_t STACK_MAX = 128;
 constexpr uint32_t STACK_SPILL = STACK_MAX - STACK_LDS;
+// Lanes per workgroup of the traversal kernels (trace, first hits).
+constexpr uint32_t TRAVERSAL_BLOCK = 256;
 
 // shade_surface launch variants: workgroup size and where the BSDF tables live.
 // The _SORTED variants order each workgroup's hits by material class first.

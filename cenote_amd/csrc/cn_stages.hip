@@ -192,16 +192,16 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
 template <bool COUNT>
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                            uint32_t range_count, uint32_t bounce) {
-    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
+    __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
     // Bounce 0 traces the camera paths: item i is path i.
     const uint32_t n_ray = bounce == 0 ? range_count : queue_total(counters, bounce, Q_RAY);
     const uint32_t n = n_ray + (bounce == 0 ? 0u : queue_total(counters, bounce - 1, Q_SHADOW));
     const uint32_t* rays = p.ray_queue[bounce & 1];
-    const uint32_t lanes = gridDim.x * BLOCK;
-    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x);
+    const uint32_t lanes = gridDim.x * TRAVERSAL_BLOCK;
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * TRAVERSAL_BLOCK + threadIdx.x);
     const uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
     uint32_t ray_nodes = 0, ray_tris = 0, shadow_nodes = 0, shadow_tris = 0;
-    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
+    for (uint32_t first = blockIdx.x * TRAVERSAL_BLOCK; first < n; first += lanes) {
         const uint32_t i = first + threadIdx.x;
         const bool is_ray = i < n_ray;
         uint32_t path = 0, cls = 0;
@@ -696,15 +696,15 @@ __global__ void k_sample_sequence(uint32_t pixel, uint32_t dimension, uint32_t c
 }
 
 // The bounce-0 closest hit per pixel, for the first-hit identity check.
-__global__ __launch_bounds__(BLOCK) void k_first_hits(SceneView s, CameraView cam, PathPool p, uint32_t width,
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_first_hits(SceneView s, CameraView cam, PathPool p, uint32_t width,
                                                       uint32_t height, uint32_t sample, uint32_t* inst_out,
                                                       uint32_t* prim_out, float* bary_out) {
-    __shared__ uint32_t lds_stack[STACK_LDS * BLOCK];
+    __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
     const uint32_t n = width * height;
-    const uint32_t lanes = gridDim.x * BLOCK;
-    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * BLOCK + threadIdx.x);
+    const uint32_t lanes = gridDim.x * TRAVERSAL_BLOCK;
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * TRAVERSAL_BLOCK + threadIdx.x);
     uint32_t nodes = 0, tris = 0;
-    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += lanes) {
+    for (uint32_t first = blockIdx.x * TRAVERSAL_BLOCK; first < n; first += lanes) {
         const uint32_t pixel = first + threadIdx.x;
         if (pixel >= n) continue;
         v3 o, d;
@@ -777,10 +777,10 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     // last bounce shades terminally and leaves no shadow rays behind.
     if (prof) prof->begin(K_TRACE);
     if (counting)
-        hipLaunchKernelGGL(k_trace<true>, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count,
+        hipLaunchKernelGGL(k_trace<true>, dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, range_count,
                            bounce);
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(trace_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, range_count,
+        hipLaunchKernelGGL(k_trace<false>, dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, range_count,
                            bounce);
     if (prof) prof->end(K_TRACE);
     if (prof) prof->begin(K_SHADE_MISS);
@@ -839,7 +839,7 @@ hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t d
 hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const CameraView& cam,
                              const PathPool& p, uint32_t width, uint32_t height, uint32_t sample, uint32_t* inst,
                              uint32_t* prim, float* bary) {
-    hipLaunchKernelGGL(k_first_hits, dim3(shape.trace_blocks), dim3(BLOCK), 0, stream, s, cam, p, width, height, sample,
+    hipLaunchKernelGGL(k_first_hits, dim3(shape.trace_blocks), dim3(TRAVERSAL_BLOCK), 0, stream, s, cam, p, width, height, sample,
                        inst, prim, bary);
     return hipGetLastError();
 }

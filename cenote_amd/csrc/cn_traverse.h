@@ -16,7 +16,6 @@
 
 namespace cn {
 
-constexpr uint32_t TRAVERSAL_BLOCK = 256;
 constexpr uint32_t TRANSPARENT_CAP = 8;     // intersect.slang:23
 constexpr uint32_t ATTENUATION_CROSSINGS = 16;
 constexpr uint32_t NODE_DONE = CHILD_EMPTY;  // never pushed: empty slots fail every box test
