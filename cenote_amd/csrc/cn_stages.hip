@@ -271,7 +271,11 @@ __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, W
             const bool cameraRay = thr.w == 0.0f;
             if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
             const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
-            add_pixel(tg.albedo, pixel, guide, 0.0f);
+            // A path past its first diffuse vertex carries a zero guide.
+            // Adding zeros leaves the sum's bits as they are (it starts at
+            // +0 and never turns -0), so skip that read-modify-write; any
+            // other guide, NaN included, is added as before.
+            if (!(guide.x == 0.0f && guide.y == 0.0f && guide.z == 0.0f)) add_pixel(tg.albedo, pixel, guide, 0.0f);
         }
         const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
         add_pixel(tg.radiance, pixel, contribution, 1.0f);
