@@ -151,7 +151,7 @@ def batched_bench(args):
     1..16/9, 1..64 spp): the plain form — view after view, one sample per
     wave from the reference's 2^20-path pool, a synchronous wave each (the
     reference's Renderer::accumulate loop) — against one many-view batch
-    (shared waves from the 2^26-path pool, launch shape from the
+    (shared waves from the 2^27-path pool, launch shape from the
     autotuner). Both forms render bitwise the same films."""
     import cenote_amd as cn
     from cenote_amd import autotune, workloads

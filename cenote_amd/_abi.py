@@ -33,7 +33,7 @@ BVH_DEVICE = 0
 BVH_HOST = 1
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
-POOL_CAPACITY = 1 << 26
+POOL_CAPACITY = 1 << 27
 
 f3 = C.c_float * 3
 

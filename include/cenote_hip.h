@@ -66,11 +66,11 @@ enum {
 /* Wavefront::DEFAULT_CAPACITY and DEFAULT_MAX_BOUNCES (wavefront.rs:462-471). */
 #define CN_DEFAULT_CAPACITY (1u << 20)
 #define CN_DEFAULT_MAX_BOUNCES 8u
-/* The most paths a capacity of 0 lets a wave hold here: 2^26 (228 bytes of
- * path state and slot targets each, at most 15.3 GB, allocated as views
- * need it), so that one wave carries 32 samples of a 1080p view and even
+/* The most paths a capacity of 0 lets a wave hold here: 2^27 (228 bytes of
+ * path state and slot targets each, at most 30.6 GB, allocated as views
+ * need it), so that one wave carries 64 samples of a 1080p view and even
  * the deep-bounce launches fill the chip. */
-#define CN_POOL_CAPACITY (1u << 26)
+#define CN_POOL_CAPACITY (1u << 27)
 /* The largest pool a renderer may ask for. */
 #define CN_POOL_CAPACITY_MAX (1u << 27)
 
