@@ -111,18 +111,24 @@ __host__ __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint32_t 
 }
 
 // rng.slang:100-111, axis 1: the generator matrix is Pascal's triangle mod
-// 2, so its column for index bit k is the row-k binomial parities placed
-// from the top bit down — built by the recurrence c_k = c_{k-1} ^
-// (c_{k-1} >> 1) starting at c_0 = 0x80000000, one shift and xor per bit,
-// with no table read.
+// 2. Its column for index bit k is row k's binomial parities from the top
+// bit down, the coefficients of (1 + z)^k with z^j at bit 31 - j, so the
+// value is the index polynomial f(t) = sum of t^k over its set bits taken
+// at t = 1 + z, over GF(2). Since (1 + z)^(2^m) = 1 + z^(2^m) there,
+// splitting f into halves gives f(1 + z) = (f_lo + f_hi)(1 + z) + z^h
+// f_hi(1 + z): five steps of "each block's low half ^= its high half", at
+// block sizes 2, 4, ..., 32, then a bit reversal for the top-down order —
+// equal to the reference's bit-by-bit column loop for every 32-bit index
+// (checked exhaustively), in 16 operations instead of up to 32 dependent
+// loop trips.
 __host__ __device__ __forceinline__ uint32_t sobol_axis1(uint32_t index) {
-    uint32_t value = 0;
-    uint32_t column = 0x80000000u;
-    for (; index != 0; index >>= 1) {
-        if (index & 1u) value ^= column;
-        column ^= column >> 1;
-    }
-    return value;
+    uint32_t w = index;
+    w ^= (w >> 1) & 0x55555555u;
+    w ^= (w >> 2) & 0x33333333u;
+    w ^= (w >> 4) & 0x0f0f0f0fu;
+    w ^= (w >> 8) & 0x00ff00ffu;
+    w ^= (w >> 16) & 0x0000ffffu;
+    return bit_reverse(w);
 }
 
 // rng.slang:114-116
