@@ -29,6 +29,9 @@ namespace cn {
 constexpr uint32_t BLOCK = 256;
 constexpr uint32_t ROULETTE_START = 3;   // shade_surface.slang:53
 constexpr uint32_t MATERIAL_CLASSES = 32;  // material_class() values: 5 bits
+// A camera path's throughput: unit weight, previous pdf 0, no scatter yet
+// (raygen.slang:110).
+__device__ __forceinline__ float4 camera_throughput() { return make_float4(1.0f, 1.0f, 1.0f, 0.0f); }
 
 // --- wave64 queue aggregation ------------------------------------------------
 
@@ -165,7 +168,8 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     camera_ray(job.cam, job.width, job.height, pixel, sample, &o, &d, &cf);
     p.origin[i] = F4(o, __uint_as_float(job.max_bounces));  // w: the path's depth cap
     p.direction[i] = F4(d, cf);
-    p.throughput[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    // A camera path's throughput, (1, 1, 1) with no previous pdf, is not
+    // stored: the bounce-0 kernels use camera_throughput().
     p.key[i] = make_uint2(pixel, sample);
     tg.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (tg.albedo) {
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, W
         const uint32_t path = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
         const uint32_t pixel = base + path;
         const v3 direction = xyz(p.direction[path]);
-        const float4 thr = p.throughput[path];
+        const float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
         float weight = 1.0f;
         if (thr.w > 0.0f) {
             if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
@@ -421,7 +425,7 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             const InstanceRecord inst = load_instance(s.instances, hit.x);
             const Mat material = load_material(s.materials, hit.x);
             const Spawn spawn = spawn_point(s, inst, hit.y, __uint_as_float(hit.z), __uint_as_float(hit.w), direction);
-            float4 thr = p.throughput[path];
+            float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
 
             uint32_t medium = bounce == 0 ? MEDIUM_NONE : (p.state[path] & 0xffffffu);
             if (medium != MEDIUM_NONE) {
