@@ -381,6 +381,8 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
     }
     const bool aovOn = tg.albedo != nullptr;
     uint32_t* next_rays = p.ray_queue[(bounce + 1) & 1];
+    // Sorted variants: the lane that held each sorted hit before the sort.
+    __shared__ uint32_t sorted_home[SORT ? SB : 1];
 
     for (uint32_t first = blockIdx.x * SB; first < n; first += gridDim.x * SB) {
         const uint32_t i = first + threadIdx.x;
@@ -406,6 +408,7 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             }
             __syncthreads();
             sorted[bucket[cls] + rank] = valid ? path : 0xffffffffu;
+            sorted_home[bucket[cls] + rank] = threadIdx.x;  // read back after shading
             __syncthreads();
             path = sorted[threadIdx.x];
             valid = path != 0xffffffffu;
@@ -585,6 +588,16 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                 }
                 add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
             }
+        }
+        if constexpr (SORT) {
+            // The next bounce's rays go out in the hits' queue order, not in
+            // material order: consecutive lanes of the next kernels then
+            // read the records of nearby paths.
+            __shared__ uint32_t pushed[SB];
+            pushed[sorted_home[threadIdx.x]] = push_ray ? path : 0xffffffffu;
+            __syncthreads();
+            path = pushed[threadIdx.x];
+            push_ray = path != 0xffffffffu;
         }
         const uint32_t split = chunk_split(first);
         const uint32_t rs = wave_claim(&counters[counter_index(bounce + 1, Q_RAY, split)], push_ray);
