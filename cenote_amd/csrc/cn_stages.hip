@@ -96,6 +96,16 @@ __device__ __forceinline__ void add_pixel(float4* target, uint32_t pixel, v3 c, 
     target[pixel] = v;
 }
 
+// A slot's accumulators are not zero-filled by raygen: the bounce-0 kernel
+// that takes the camera path (shade_miss or shade_surface, exactly one of
+// them) makes each target's first write a store of what adding to zeros
+// would give (0 + c, which also turns a -0 into +0 as the add did), and
+// stores zeros where it has nothing to add.
+__device__ __forceinline__ void put_pixel(float4* target, uint32_t pixel, v3 c, float a, bool first) {
+    if (first) target[pixel] = make_float4(0.0f + c.x, 0.0f + c.y, 0.0f + c.z, 0.0f + a);
+    else add_pixel(target, pixel, c, a);
+}
+
 
 // --- raygen (raygen.slang:64-112) --------------------------------------------------
 
@@ -142,9 +152,10 @@ __device__ __forceinline__ void camera_ray(const CameraView& cam, uint32_t width
 
 // One thread per slot of the range: find the slot's job (jobs sorted by
 // slot base), derive its pixel and sample, and start the camera path. Also
-// zero-fills the slot's accumulators (the reference's Pass::Fill,
-// wavefront.rs:733-759) and resets the range's queue counters (one word per
-// 128-byte counter line): raygen owns every slot exactly once.
+// resets the range's queue counters (one word per 128-byte counter line):
+// raygen owns every slot exactly once. The reference's Pass::Fill
+// (wavefront.rs:733-759) zero-fills the slot's accumulators; here the
+// bounce-0 kernels store their first values instead (put_pixel).
 __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ jobs, uint32_t job_count, PathPool p,
                                                   WaveTargets tg, uint32_t* counters, uint32_t counter_lines,
                                                   uint32_t base, uint32_t count) {
@@ -171,12 +182,6 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     // A camera path's throughput, (1, 1, 1) with no previous pdf, is not
     // stored: the bounce-0 kernels use camera_throughput().
     p.key[i] = make_uint2(pixel, sample);
-    tg.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (tg.albedo) {
-        tg.albedo[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        tg.normal[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        tg.depth[slot] = 0.0f;
-    }
 }
 
 // --- trace: intersect (intersect.slang:45-88) fused with the previous
@@ -273,16 +278,23 @@ __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, W
         }
         if (tg.albedo) {
             const bool cameraRay = thr.w == 0.0f;
-            if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
-            const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
-            // A path past its first diffuse vertex carries a zero guide.
-            // Adding zeros leaves the sum's bits as they are (it starts at
-            // +0 and never turns -0), so skip that read-modify-write; any
-            // other guide, NaN included, is added as before.
-            if (!(guide.x == 0.0f && guide.y == 0.0f && guide.z == 0.0f)) add_pixel(tg.albedo, pixel, guide, 0.0f);
+            if (bounce == 0) {
+                // A camera path (thr.w == 0): the slot's first AOV writes.
+                tg.depth[pixel] = 0.0f + __uint_as_float(0x7f800000u);
+                put_pixel(tg.albedo, pixel, S3(1.0f), 0.0f, true);
+                tg.normal[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            } else {
+                if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
+                const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
+                // A path past its first diffuse vertex carries a zero guide.
+                // Adding zeros leaves the sum's bits as they are (it starts
+                // at +0 and never turns -0), so skip that read-modify-write;
+                // any other guide, NaN included, is added as before.
+                if (!(guide.x == 0.0f && guide.y == 0.0f && guide.z == 0.0f)) add_pixel(tg.albedo, pixel, guide, 0.0f);
+            }
         }
         const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
-        add_pixel(tg.radiance, pixel, contribution, 1.0f);
+        put_pixel(tg.radiance, pixel, contribution, 1.0f, bounce == 0);
     }
 }
 
@@ -366,7 +378,7 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
 // workgroup shades its hits reaches no result: every add lands in the
 // path's own slot.
 template <uint32_t SB, bool LDS_TABLES, bool SORT>
-__global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+__global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_shade_surface(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
     const uint32_t n = queue_total(counters, bounce, Q_HIT);
     if (blockIdx.x * SB >= n) return;  // uniform per block: nothing to shade here
@@ -464,7 +476,8 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             // known; a terminal path adds its sample count (and an emission
             // that was not positive) at the end. A slot sees the same sums.
             const bool lit = any_positive(contribution);
-            if (lit) add_pixel(tg.radiance, pixel, contribution, 0.0f);
+            if (bounce == 0) put_pixel(tg.radiance, pixel, lit ? contribution : S3(0.0f), 0.0f, true);
+            else if (lit) add_pixel(tg.radiance, pixel, contribution, 0.0f);
 
             // The next-event candidate (a point on a light, or an
             // environment direction) depends only on the hit point, so it is
@@ -498,7 +511,7 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             v3 guide = S3(0.0f);
             if (aovOn) {
                 guide = bounce == 0 ? S3(1.0f) : xyz(tg.guide[pixel]);
-                if (bounce == 0) tg.depth[pixel] += length(origin - spawn.position) * dir4.w;
+                if (bounce == 0) tg.depth[pixel] = 0.0f + length(origin - spawn.position) * dir4.w;
             }
             const v3 aovAlbedo = vclamp(closure.aovAlbedo, 0.0f, 1.0f);
             bool terminal = false;
@@ -565,8 +578,11 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
                         const float record = aov_record_fraction(closure, scatter.lobe);
                         if (record > 0.0f && any_positive(guide)) {
                             const v3 g = guide * record;
-                            add_pixel(tg.albedo, pixel, g * aovAlbedo, 0.0f);
-                            add_pixel(tg.normal, pixel, g * shading, 0.0f);
+                            put_pixel(tg.albedo, pixel, g * aovAlbedo, 0.0f, bounce == 0);
+                            put_pixel(tg.normal, pixel, g * shading, 0.0f, bounce == 0);
+                        } else if (bounce == 0) {
+                            put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
+                            put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
                         }
                         tg.guide[pixel] = F4((guide * (1.0f - record)) * aov_pass_tint(closure, scatter.lobe), 0.0f);
                     }
@@ -583,8 +599,11 @@ __global__ __launch_bounds__(SB) void k_shade_surface(SceneView s, PathPool p, W
             }
             if (terminal) {
                 if (aovOn && any_positive(guide)) {
-                    add_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f);
-                    add_pixel(tg.normal, pixel, guide * shading, 0.0f);
+                    put_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f, bounce == 0);
+                    put_pixel(tg.normal, pixel, guide * shading, 0.0f, bounce == 0);
+                } else if (aovOn && bounce == 0) {
+                    put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
+                    put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
                 }
                 add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
             }
