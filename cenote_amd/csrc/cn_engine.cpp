@@ -536,10 +536,24 @@ int cn_trace_aov(cn_renderer* r, cn_scene* s, float* radiance, float* albedo, fl
     if (int e = scene_camera(s, s->camera, width, height, &job.cam)) return e;
     if (int e = enqueue_wave(r, {job}, aov)) return e;
     CN_HIP(hipStreamSynchronize(r->stream));
-    if (radiance) CN_HIP(hipMemcpy(radiance, r->radiance.ptr, pixels * 16, hipMemcpyDeviceToHost));
-    if (albedo) CN_HIP(hipMemcpy(albedo, r->albedo.ptr, pixels * 16, hipMemcpyDeviceToHost));
-    if (normal) CN_HIP(hipMemcpy(normal, r->normal.ptr, pixels * 16, hipMemcpyDeviceToHost));
-    if (depth) CN_HIP(hipMemcpy(depth, r->depth.ptr, pixels * 4, hipMemcpyDeviceToHost));
+    // The slots hold the view in band order (band_pixel); the images are
+    // row-major.
+    auto fetch = [&](void* out, const void* slots, size_t bytes) -> int {
+        std::vector<unsigned char> tmp(pixels * bytes);
+        CN_HIP(hipMemcpy(tmp.data(), slots, tmp.size(), hipMemcpyDeviceToHost));
+        unsigned char* dst = static_cast<unsigned char*>(out);
+        for (uint32_t i = 0; i < pixels; ++i)
+            std::memcpy(dst + size_t(band_pixel(i, width, height)) * bytes, tmp.data() + size_t(i) * bytes, bytes);
+        return CN_OK;
+    };
+    if (radiance)
+        if (int e = fetch(radiance, r->radiance.ptr, 16)) return e;
+    if (albedo)
+        if (int e = fetch(albedo, r->albedo.ptr, 16)) return e;
+    if (normal)
+        if (int e = fetch(normal, r->normal.ptr, 16)) return e;
+    if (depth)
+        if (int e = fetch(depth, r->depth.ptr, 4)) return e;
     return CN_OK;
 }
 

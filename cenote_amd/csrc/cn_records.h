@@ -171,6 +171,28 @@ struct SceneView {
     uint32_t env_conditional_buckets;
 };
 
+// The order of a view's camera paths in the slot space: 8-row bands top to
+// bottom, each walked column by column, so a wave's 64 consecutive slots
+// cover an 8 x 8 block of pixels (fewer rows in the last band of a height
+// that is not a multiple of 8). Compact blocks keep a wave's camera rays,
+// and the hits and rays that follow them, close together in the BVH and in
+// memory. band_pixel maps a slot-order index to its pixel, band_index back.
+__host__ __device__ inline uint32_t band_pixel(uint32_t index, uint32_t width, uint32_t height) {
+    const uint32_t band = index / (width * 8u);
+    const uint32_t j = index - band * (width * 8u);
+    const uint32_t left = height - band * 8u;
+    const uint32_t rows = left < 8u ? left : 8u;
+    const uint32_t x = j / rows;
+    return (band * 8u + (j - x * rows)) * width + x;
+}
+__host__ __device__ inline uint32_t band_index(uint32_t pixel, uint32_t width, uint32_t height) {
+    const uint32_t y = pixel / width, x = pixel - y * width;
+    const uint32_t band = y / 8u;
+    const uint32_t left = height - band * 8u;
+    const uint32_t rows = left < 8u ? left : 8u;
+    return band * (width * 8u) + x * rows + (y - band * 8u);
+}
+
 // The SoA path pool (shaders/pathstate.slang:207-232). A wave is a set of
 // jobs — (view, consecutive samples) pairs — laid end to end in a slot
 // space; a path's target slot is its range base plus its pool index, and

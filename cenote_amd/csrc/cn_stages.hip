@@ -172,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     const JobRecord& job = jobs[lo];
     const uint32_t local = slot - job.slot_base;
     const uint32_t k = local / job.pixels;
-    const uint32_t pixel = local - k * job.pixels;
+    const uint32_t pixel = band_pixel(local - k * job.pixels, job.width, job.height);
     const uint32_t sample = job.first_sample + k;
     v3 o, d;
     float cf;
@@ -647,7 +647,10 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restric
         }
         const JobRecord& job = jobs[lo];
         if (!job.beauty_sum) continue;
-        const uint32_t px = i - job.pixel_base;
+        // i walks the view in slot order (the slot reads coalesce); px is
+        // the film pixel that slot order index holds.
+        const uint32_t index = i - job.pixel_base;
+        const uint32_t px = band_pixel(index, job.width, job.height);
         const bool reset = job.flags & JOB_RESET;
         float4 b = reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : job.beauty_sum[px];
         const bool aov = tg.albedo != nullptr && job.albedo_sum != nullptr;
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restric
             d = job.depth_sum[px];
         }
         for (uint32_t k = 0; k < job.samples; ++k) {
-            const uint32_t slot = job.slot_base + k * job.pixels + px;
+            const uint32_t slot = job.slot_base + k * job.pixels + index;
             const bool first = reset && k == 0;
             const float4 cb = guarded(tg.radiance[slot]);
             b = first ? cb : add4(b, cb);
