@@ -347,9 +347,9 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     const uint32_t cus = std::max(1u, ctx->compute_units);
     // Defaults from the autotuner's sweeps on the chart and heavy scenes;
     // cn_renderer_set_shape overrides them.
-    r->shape.trace_blocks = cus * 32;
+    r->shape.trace_blocks = cus * 64;
     r->shape.shade_variant = SHADE_256_GLOBAL_SORTED;
-    r->shape.shade_blocks = cus * 8;
+    r->shape.shade_blocks = cus * 16;
     r->shape.light_blocks = cus * 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * TRAVERSAL_BLOCK * STACK_SPILL));
