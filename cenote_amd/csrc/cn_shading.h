@@ -515,16 +515,96 @@ __device__ inline float bsdf_pdf(const Closure& c, v3 wi, v3 wo) {
     return pdf / c.wTotal;
 }
 
+// f(wi, wo) and pdf(wi | wo) together: eval_bsdf's and bsdf_pdf's values
+// bit for bit, with the terms both compute for a reflected direction (the
+// half vector, each lobe's D and G1(wo), the dielectric Fresnel, the sheen
+// density) computed once, under the union of the two functions' conditions
+// and by the same expressions (products reordered only where IEEE
+// multiplication commutes). A transmitted direction takes the two
+// functions as they are.
+struct BsdfEval {
+    v3 f;
+    float pdf;
+};
+
+__device__ inline BsdfEval eval_with_pdf(const Closure& c, v3 wi, v3 wo) {
+    BsdfEval r{S3(0.0f), 0.0f};
+    if (wo.z <= 0.0f || wi.z == 0.0f) return r;
+    if (wi.z < 0.0f) {
+        r.f = eval_bsdf(c, wi, wo);
+        r.pdf = bsdf_pdf(c, wi, wo);
+        return r;
+    }
+    const Mat& m = c.m;
+    const bool pdfOn = !(c.wTotal <= 0.0f);  // bsdf_pdf's own early exit
+    const bool fuzzF = c.fuzzR > 0.0f, fuzzP = pdfOn && c.wFuzz > 0.0f;
+    const bool coatF = m.coatWeight > 0.0f && !c.exiting, coatP = pdfOn && c.wCoat > 0.0f;
+    const bool specF = c.hasSpecular, specP = pdfOn && c.wSpec > 0.0f;
+    const float density = (fuzzF || fuzzP) ? sheen_density(c, align_to_view(wi, wo, false)) : 0.0f;
+    const v3 h = normalize(wi + wo);
+    const float cosVH = dot(h, wo);
+    float coatD = 0.0f, coatG1o = 0.0f, specD = 0.0f, specG1o = 0.0f, fd = 0.0f;
+    if (coatF || coatP) {
+        coatD = ggx_d(c.coatAlpha, h.z);
+        coatG1o = smith_g1(c.coatAlpha, wo.z);
+    }
+    if (specF || specP) {
+        specD = ggx_d(c.alpha, h.z);
+        specG1o = smith_g1(c.alpha, wo.z);
+        fd = fresnel_dielectric(c.eta, cosVH);
+    }
+
+    // eval_bsdf's reflection branch
+    v3 result = S3(0.0f);
+    if (fuzzF) result = result + ((m.fuzzWeight * m.fuzzColor) * c.fuzzR) * density / wi.z;
+    if (coatF) {
+        const float coatFresnel = fresnel_dielectric(m.coatIor, cosVH);
+        const float core = coatD * coatG1o * smith_g1(c.coatAlpha, wi.z) / (4.0f * wo.z * wi.z);
+        result = result + S3(c.fuzzAtten * m.coatWeight * coatFresnel * c.coatScale * core);
+    }
+    v3 base = S3(0.0f);
+    if (specF) {  // base_reflection
+        const float core = specD * specG1o * smith_g1(c.alpha, wi.z) / (4.0f * wo.z * wi.z);
+        if (c.exiting) {
+            base = S3(core * fd) * c.glassScale;
+        } else {
+            const v3 conductor = fresnel_schlick(m.baseColor, cosVH) * c.conductorScale;
+            v3 dielectric = m.thinWalled != 0
+                                ? vlerp(S3(fd * c.dielectricScale), c.thinR * c.thinScaleR, m.transmissionWeight)
+                                : vlerp(S3(fd * c.dielectricScale), fd * c.glassScale, m.transmissionWeight);
+            base = core * vlerp(dielectric, conductor, m.metalness);
+        }
+    }
+    if (!c.exiting) {
+        base = base + (((1.0f - m.metalness) * (1.0f - m.transmissionWeight) * (1.0f - c.specE)) *
+                       eon_brdf(m.baseColor, m.baseRoughness, wi, wo));
+    }
+    r.f = result + c.baseWeight * base;
+
+    // bsdf_pdf's reflection branch
+    if (pdfOn) {
+        float pdf = 0.0f;
+        if (c.wFuzz > 0.0f) pdf += c.wFuzz * density;
+        if (c.wCoat > 0.0f) pdf += c.wCoat * (coatG1o * coatD / (4.0f * wo.z));
+        if (c.wSpec > 0.0f) pdf += c.wSpec * (specG1o * specD / (4.0f * wo.z)) * reflect_probability(c, fd);
+        if (c.wDiffuse > 0.0f) pdf += c.wDiffuse * wi.z / PI_B;
+        r.pdf = pdf / c.wTotal;
+    }
+    return r;
+}
+
 struct BsdfSample {
     v3 wi;
     float pdf;
     uint32_t lobe;
 };
 
-// openpbr.slang:821-884
-__device__ inline BsdfSample sample_bsdf(const Closure& c, v3 wo, float uLobe, f2 u) {
-    BsdfSample r{V3(0.0f, 0.0f, 1.0f), 0.0f, LOBE_DIFFUSE};
-    if (c.wTotal <= 0.0f) return r;
+// openpbr.slang:821-884, up to the pdf: draws the scattered direction and
+// lobe into `r` (pdf left 0) and returns whether one was drawn; the pdf of a
+// drawn direction is bsdf_pdf(c, r.wi, wo), which eval_with_pdf also gives.
+__device__ inline bool sample_bsdf_direction(const Closure& c, v3 wo, float uLobe, f2 u, BsdfSample& r) {
+    r = BsdfSample{V3(0.0f, 0.0f, 1.0f), 0.0f, LOBE_DIFFUSE};
+    if (c.wTotal <= 0.0f) return false;
     const float pick = uLobe * c.wTotal;
     v3 wi;
     if (pick < c.wFuzz) {
@@ -563,7 +643,7 @@ __device__ inline BsdfSample sample_bsdf(const Closure& c, v3 wo, float uLobe, f
                 // refractDirection, openpbr.slang:270-277
                 const float cc = dot(wo, h);
                 const float cos2t = 1.0f - (1.0f - cc * cc) / (c.eta * c.eta);
-                if (cos2t <= 0.0f) return r;
+                if (cos2t <= 0.0f) return false;
                 wi = h * (cc / c.eta - sqrtf(cos2t)) - wo / c.eta;
             }
         }
@@ -575,8 +655,7 @@ __device__ inline BsdfSample sample_bsdf(const Closure& c, v3 wo, float uLobe, f
         wi = V3(rr * cosf(phi), rr * sinf(phi), sqrtf(1.0f - u.x));
     }
     r.wi = wi;
-    r.pdf = bsdf_pdf(c, wi, wo);
-    return r;
+    return true;
 }
 
 // Duff et al. orthonormal basis (openpbr.slang:888-909).

@@ -528,12 +528,12 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     const bool through = dot(cand.direction, spawn.normal) < 0.0f;
                     if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
                         const float neePdf = (cand_env ? envProb : 1.0f - envProb) * cand.pdf;
+                        const BsdfEval light = eval_with_pdf(closure, wiLight, wo);
                         const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
-                                                 ? power_heuristic(neePdf, bsdf_pdf(closure, wiLight, wo))
+                                                 ? power_heuristic(neePdf, light.pdf)
                                                  : 1.0f;
                         const v3 unshadowed =
-                            (((T * cand.radiance) * eval_bsdf(closure, wiLight, wo)) * fabsf(wiLight.z)) *
-                            (weight / neePdf);
+                            (((T * cand.radiance) * light.f) * fabsf(wiLight.z)) * (weight / neePdf);
                         sh_origin = through ? spawn.below : spawn.position;
                         if (any_positive(unshadowed)) {
                             push_shadow = true;
@@ -556,12 +556,15 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 // Scatter (shade_surface.slang:453-489).
                 const float uLobe = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_LOBE));
                 const f2 u = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_DIR));
-                const BsdfSample scatter = sample_bsdf(closure, wo, uLobe, u);
+                BsdfSample scatter;
+                BsdfEval drawn{S3(0.0f), 0.0f};
+                if (sample_bsdf_direction(closure, wo, uLobe, u, scatter)) drawn = eval_with_pdf(closure, scatter.wi, wo);
+                scatter.pdf = drawn.pdf;
                 const v3 wiWorld = to_world(frame, scatter.wi);
                 v3 scattered = S3(0.0f);
                 if (scatter.pdf > 0.0f && scatter.wi.z != 0.0f &&
                     (scatter.wi.z > 0.0f) == (dot(wiWorld, spawn.normal) > 0.0f)) {
-                    scattered = (T * eval_bsdf(closure, scatter.wi, wo)) * (fabsf(scatter.wi.z) / scatter.pdf);
+                    scattered = (T * drawn.f) * (fabsf(scatter.wi.z) / scatter.pdf);
                 }
                 float survival = fmaxf(scattered.x, fmaxf(scattered.y, scattered.z));
                 if (survival <= 0.0f) {
