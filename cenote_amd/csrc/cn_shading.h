@@ -520,8 +520,8 @@ __device__ inline float bsdf_pdf(const Closure& c, v3 wi, v3 wo) {
 // half vector, each lobe's D and G1(wo), the dielectric Fresnel, the sheen
 // density) computed once, under the union of the two functions' conditions
 // and by the same expressions (products reordered only where IEEE
-// multiplication commutes). A transmitted direction takes the two
-// functions as they are.
+// multiplication commutes); for a transmitted direction, the refraction
+// half vector, Fresnel, D and G1(wo) likewise.
 struct BsdfEval {
     v3 f;
     float pdf;
@@ -530,13 +530,50 @@ struct BsdfEval {
 __device__ inline BsdfEval eval_with_pdf(const Closure& c, v3 wi, v3 wo) {
     BsdfEval r{S3(0.0f), 0.0f};
     if (wo.z <= 0.0f || wi.z == 0.0f) return r;
-    if (wi.z < 0.0f) {
-        r.f = eval_bsdf(c, wi, wo);
-        r.pdf = bsdf_pdf(c, wi, wo);
-        return r;
-    }
     const Mat& m = c.m;
     const bool pdfOn = !(c.wTotal <= 0.0f);  // bsdf_pdf's own early exit
+    if (wi.z < 0.0f) {  // both functions' transmission branches
+        const float glassShare = c.exiting ? 1.0f : (1.0f - m.metalness) * m.transmissionWeight;
+        const bool fOn = !(glassShare <= 0.0f || (c.exiting && m.thinWalled != 0));
+        const bool pOn = pdfOn && !(c.wSpec <= 0.0f);
+        if (!fOn && !pOn) return r;
+        if (m.thinWalled != 0) {
+            const v3 flipped = V3(wi.x, wi.y, -wi.z);
+            const v3 h = normalize(flipped + wo);
+            const float D = ggx_d(c.thinAlpha, h.z), G1o = smith_g1(c.thinAlpha, wo.z);
+            if (fOn) {
+                const float core = D * G1o * smith_g1(c.thinAlpha, flipped.z) / (4.0f * wo.z * flipped.z);
+                r.f = ((c.baseWeight * glassShare) * c.thinT) * c.thinScaleT * core;
+            }
+            if (pOn) {
+                const float pdf = G1o * D / (4.0f * wo.z) * (1.0f - reflect_probability(c, 0.0f));
+                r.pdf = c.wSpec * pdf / c.wTotal;
+            }
+            return r;
+        }
+        const v3 hU = wo + c.eta * wi;
+        const float len2 = dot(hU, hU);
+        if (len2 < 1e-9f) return r;
+        v3 h = hU / sqrtf(len2);
+        h = h.z >= 0.0f ? h : -h;
+        const float cosVH = dot(h, wo);
+        const float cosLH = dot(h, wi);
+        if (cosVH <= 0.0f || cosLH >= 0.0f) return r;
+        const float f = fresnel_dielectric(c.eta, cosVH);
+        const float D = ggx_d(c.alpha, h.z), G1o = smith_g1(c.alpha, wo.z);
+        if (fOn) {
+            const float core = D * G1o * smith_g1(c.alpha, wi.z) * cosVH * (c.eta * c.eta) * fabsf(cosLH) /
+                               (wo.z * fabsf(wi.z) * len2);
+            r.f = ((((c.baseWeight * glassShare) * (1.0f - f)) * c.glassTint) * c.glassScale) * core;
+        }
+        if (pOn) {
+            const float vndf = G1o * D * cosVH / wo.z;
+            const float jacobian = (c.eta * c.eta) * fabsf(cosLH) / len2;
+            const float pdf = vndf * jacobian * (1.0f - reflect_probability(c, f));
+            r.pdf = c.wSpec * pdf / c.wTotal;
+        }
+        return r;
+    }
     const bool fuzzF = c.fuzzR > 0.0f, fuzzP = pdfOn && c.wFuzz > 0.0f;
     const bool coatF = m.coatWeight > 0.0f && !c.exiting, coatP = pdfOn && c.wCoat > 0.0f;
     const bool specF = c.hasSpecular, specP = pdfOn && c.wSpec > 0.0f;
