@@ -722,8 +722,16 @@ __device__ __forceinline__ v3 env_bilinear(const SceneView& s, float u, float v)
     const float ax = x - fx, ay = y - fy;
     int x0 = int(fx), y0 = int(fy);
     int x1 = x0 + 1, y1 = y0 + 1;
-    x0 = ((x0 % w) + w) % w;
-    x1 = ((x1 % w) + w) % w;
+    // For u in [0, 1], x0 lies in [-1, w - 1] and x1 in [0, w]: one add or
+    // subtract wraps them exactly as ((x % w) + w) % w does, without its two
+    // integer divisions; any other value (a non-finite u) takes that form.
+    if (x0 >= -1 && x0 < w) {
+        x0 = x0 < 0 ? x0 + w : x0;
+        x1 = x1 >= w ? x1 - w : x1;
+    } else {
+        x0 = ((x0 % w) + w) % w;
+        x1 = ((x1 % w) + w) % w;
+    }
     y0 = min(max(y0, 0), h - 1);
     y1 = min(max(y1, 0), h - 1);
     const v3 t00 = xyz(s.env_texels[y0 * w + x0]);
