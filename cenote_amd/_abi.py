@@ -34,6 +34,7 @@ BVH_HOST = 1
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
 POOL_CAPACITY = 1 << 27
+STATIC_MAX_PIXELS = 1 << 18
 
 f3 = C.c_float * 3
 
@@ -305,6 +306,7 @@ SIGNATURES = {
     "cn_film_samples": [_P, _UP],
     "cn_accumulate": [_P, _P, _P],
     "cn_accumulate_async": [_P, _P, _P, C.c_uint32],
+    "cn_accumulate_async_static": [_P, _P, _P, C.c_uint32],
     "cn_render_batch": [_P, C.POINTER(ViewJob), C.c_uint32],
     "cn_server_create": [_P, C.c_uint64, C.c_uint32, C.POINTER(_P)],
     "cn_server_destroy": [_P],

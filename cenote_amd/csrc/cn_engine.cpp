@@ -128,7 +128,7 @@ int ensure_slots(cn_renderer* r, size_t slots, bool aov) {
     return CN_OK;
 }
 
-int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
+int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena) {
     if (jobs.empty()) return CN_OK;
     if (jobs.size() > MAX_JOBS) return fail(CN_E_INVALID, "a wave holds at most %u jobs", MAX_JOBS);
     for (HostJob& j : jobs) {
@@ -162,8 +162,36 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
         pixels += uint64_t(j.width) * j.height;
     }
     if (slots >= (1ull << 32)) return fail(CN_E_INVALID, "wave of %llu slots is too large", (unsigned long long)slots);
-    if (int e = ensure_slots(r, size_t(slots), aov)) return e;
-    if (int e = ensure_pool(r, size_t(slots))) return e;
+    PathPool pool;
+    WaveTargets tg{};
+    uint32_t capacity = r->capacity;
+    if (arena) {
+        if (slots > arena->slots)
+            return fail(CN_E_INVALID, "a wave of %llu slots exceeds the arena's %zu", (unsigned long long)slots,
+                        arena->slots);
+        pool = arena->pool;
+        pool.stack_overflow = r->pool.stack_overflow;
+        pool.traversal_counts = r->pool.traversal_counts;
+        capacity = std::min(capacity, arena->capacity);
+        tg.radiance = arena->targets.radiance;
+        if (aov) {
+            tg.albedo = arena->targets.albedo;
+            tg.normal = arena->targets.normal;
+            tg.depth = arena->targets.depth;
+            tg.guide = arena->targets.guide;
+        }
+    } else {
+        if (int e = ensure_slots(r, size_t(slots), aov)) return e;
+        if (int e = ensure_pool(r, size_t(slots))) return e;
+        pool = r->pool;
+        tg.radiance = r->radiance.ptr;
+        if (aov) {
+            tg.albedo = r->albedo.ptr;
+            tg.normal = r->normal.ptr;
+            tg.depth = r->depth.ptr;
+            tg.guide = r->guide.ptr;
+        }
+    }
 
     // A job table from the ring: wait until the wave that last used it is done.
     const uint32_t e = r->ring_next;
@@ -202,14 +230,6 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
     }
     CN_HIP(hipMemcpyAsync(dev, host, jobs.size() * sizeof(JobRecord), hipMemcpyHostToDevice, r->stream));
 
-    WaveTargets tg{};
-    tg.radiance = r->radiance.ptr;
-    if (aov) {
-        tg.albedo = r->albedo.ptr;
-        tg.normal = r->normal.ptr;
-        tg.depth = r->depth.ptr;
-        tg.guide = r->guide.ptr;
-    }
     bool films = false;
     for (const HostJob& j : jobs) films = films || j.film != nullptr;
 
@@ -228,8 +248,8 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
         for (size_t k = g0; k < g1; ++k) rounds = std::max(rounds, jobs[k].max_bounces);
         const uint32_t s0 = host[g0].slot_base;
         const uint32_t s1 = host[g1 - 1].slot_base + host[g1 - 1].pixels * host[g1 - 1].samples;
-        for (uint64_t base = s0; base < s1; base += r->capacity)
-            plan.push_back({g0, g1, uint32_t(base), uint32_t(std::min<uint64_t>(r->capacity, s1 - base)), rounds});
+        for (uint64_t base = s0; base < s1; base += capacity)
+            plan.push_back({g0, g1, uint32_t(base), uint32_t(std::min<uint64_t>(capacity, s1 - base)), rounds});
         g0 = g1;
     }
     r->last_ranges = uint32_t(plan.size());
@@ -242,10 +262,10 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
             // counter sets are reused past MAX_RANGES (only the per-wave
             // diagnostics need them distinct).
             uint32_t* counters = r->counters.ptr + (k % MAX_RANGES) * per;
-            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), r->pool, tg, counters,
+            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), pool, tg, counters,
                                  counter_lines_per_range(r->max_bounces), rg.base, rg.count, r->profiler.get()));
             for (uint32_t b = 0; b < rg.rounds; ++b) {
-                CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, r->pool, tg, counters, rg.base,
+                CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, pool, tg, counters, rg.base,
                                      rg.count, b, r->light_sampling, r->counting, r->profiler.get()));
             }
         }
@@ -260,8 +280,8 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov) {
         // Everything that reaches a kernel argument, except the job table's
         // contents (read from device memory at run time).
         std::vector<uint64_t> key = {uint64_t(jobs.size()), pixels, uint64_t(films), uint64_t(aov),
-                                     uint64_t(uintptr_t(tg.radiance)), uint64_t(uintptr_t(r->pool.origin)),
-                                     uint64_t(uintptr_t(r->pool.stack_overflow)), r->shape.trace_blocks,
+                                     uint64_t(uintptr_t(tg.radiance)), uint64_t(uintptr_t(pool.origin)),
+                                     uint64_t(uintptr_t(pool.stack_overflow)), r->shape.trace_blocks,
                                      r->shape.shade_variant, r->shape.shade_blocks, r->shape.light_blocks,
                                      r->light_sampling};
         for (const Range& rg : plan) {

@@ -139,9 +139,21 @@ struct HostJob {
     bool moments = false;      // also fold squared luminance (the film must hold a moment buffer)
 };
 
+// Device memory a wave can run in instead of the renderer's own buffers:
+// the path pool with its queues and shadow records, and the slot targets
+// (the traversal spill area and counters stay the renderer's).
+struct WaveArena {
+    PathPool pool;
+    WaveTargets targets;  // radiance, albedo, normal, depth, guide: `slots` entries each
+    uint32_t capacity;    // paths per pool range
+    size_t slots;         // (view pixel, sample) slots one wave may hold
+};
+
 // Enqueue one wave of jobs on the renderer's stream (no host sync unless a
-// buffer must grow or a job-table ring entry is still in flight).
-int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov);
+// buffer must grow or a job-table ring entry is still in flight). With an
+// arena the wave runs in it and nothing is allocated; a wave of more slots
+// than the arena holds fails with CN_E_INVALID.
+int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena = nullptr);
 
 // A film job for `samples` more samples of `camera` on `s`, continuing at
 // the film's sample count.
@@ -149,6 +161,10 @@ int film_job(cn_renderer* r, cn_scene* s, cn_film* f, const cn_camera& camera, u
 
 // Samples of a `pixels`-pixel view that fill one wave of the renderer's pool.
 uint32_t samples_per_wave(const cn_renderer* r, uint32_t pixels, uint32_t wanted);
+
+// Takes the device address of the allocation-free accumulate's arena on
+// `device` (which must be current); once per device, from cn_context_create.
+int static_arena_init(int device);
 
 }  // namespace cn
 

@@ -261,4 +261,11 @@ struct JobRecord {
 // Jobs one wave may carry.
 constexpr uint32_t MAX_JOBS = 256;
 
+// The allocation-free accumulate's device arena: waves of at most
+// STATIC_PATHS paths and slots, in one array of STATIC_SCRATCH_BYTES per
+// device (path state 76 B, queues and shadow records 68 B, slot targets
+// 68 B per path: 53 MiB).
+constexpr uint32_t STATIC_PATHS = CN_STATIC_MAX_PIXELS;
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(53) << 20;
+
 }  // namespace cn

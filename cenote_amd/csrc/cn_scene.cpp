@@ -103,6 +103,7 @@ int cn_context_create(int device, cn_context** out) {
     std::string error;
     if (!read_tables(&tables, &error)) return fail(CN_E_INVALID, "%s", error.c_str());
     CN_HIP(ctx->tables.upload(tables.data(), tables.size()));
+    if (int e = static_arena_init(device)) return e;
     *out = ctx.release();
     return CN_OK;
 }

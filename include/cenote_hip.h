@@ -373,6 +373,21 @@ int cn_accumulate(cn_renderer* renderer, cn_scene* scene, cn_film* film);
  * them; the sums are bitwise those of `count` cn_accumulate calls. */
 int cn_accumulate_async(cn_renderer* renderer, cn_scene* scene, cn_film* film,
                         uint32_t count);
+/* Largest film (width * height) cn_accumulate_async_static takes. */
+#define CN_STATIC_MAX_PIXELS (1u << 18)
+/* cn_accumulate_async without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The wave's path
+ * pool, queues and slot targets live in one library-owned device array of
+ * 53 MiB per device (taken when a context is created on the device, held
+ * until the process ends), so waves hold at most CN_STATIC_MAX_PIXELS
+ * paths and a film of more pixels fails with CN_E_INVALID. The sums are
+ * bitwise those cn_accumulate_async gives. Calls on one device share the
+ * array: calls on one renderer are ordered by its stream, but calls whose
+ * waves overlap in time on one device (two renderers' streams at once)
+ * race. The renderer, scene and film must already exist. */
+int cn_accumulate_async_static(cn_renderer* renderer, cn_scene* scene, cn_film* film,
+                               uint32_t count);
 /* One view of a batch: `samples` more samples of `camera` on `scene`,
  * folded into `film` (whose size sets the view's resolution), with paths
  * capped at `max_bounces` (0 = the renderer's; at most the renderer's). */
