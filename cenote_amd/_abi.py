@@ -31,6 +31,12 @@ LIGHT_POINT = 2
 TEXTURE_NONE = 0xFFFFFFFF
 BVH_DEVICE = 0
 BVH_HOST = 1
+DIRTY_CAMERA = 1
+DIRTY_MATERIALS = 2
+DIRTY_LIGHTS = 4
+DIRTY_ENVIRONMENT = 8
+DIRTY_GEOMETRY = 16
+DIRTY_ALL = 31
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
 POOL_CAPACITY = 1 << 27
@@ -159,6 +165,8 @@ class SceneInfo(C.Structure):
         ("bvh_builder", C.c_uint32),
         ("bvh_build_seconds", C.c_double),
         ("prep_seconds", C.c_double),
+        ("bvh8_node_count", C.c_uint32),
+        ("bvh8_max_depth", C.c_uint32),
     ]
 
 
@@ -196,6 +204,7 @@ class LaunchShape(C.Structure):
         ("shade_variant", C.c_uint32),
         ("shade_blocks", C.c_uint32),
         ("light_blocks", C.c_uint32),
+        ("bvh_width", C.c_uint32),
     ]
 
 
@@ -279,11 +288,13 @@ SIGNATURES = {
     "cn_scene_create": [_P, C.POINTER(SceneDesc), C.POINTER(_P)],
     "cn_scene_destroy": [_P],
     "cn_scene_set_camera": [_P, C.POINTER(Camera)],
+    "cn_scene_update": [_P, C.POINTER(SceneDesc), C.c_uint32],
     "cn_scene_info_get": [_P, C.POINTER(SceneInfo)],
     "cn_scene_bvh": [_P, _P, C.c_uint32, _UP, _P, C.c_uint32, _UP],
     "cn_scene_light_records": [_P, C.POINTER(LightRecord), C.c_uint32, _UP],
     "cn_scene_env_tables": [_P, _FP, _FP, _FP],
     "cn_bvh_quantize": [C.c_void_p, C.c_uint32, C.c_void_p],
+    "cn_scene_bvh8": [_P, _P, C.c_uint32, _UP],
     "cn_scene_sample_environment": [_P, _FP, C.c_uint32, _FP],
     "cn_renderer_create": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)],
     "cn_renderer_destroy": [_P],
@@ -297,6 +308,7 @@ SIGNATURES = {
     "cn_renderer_traversal_counts": [_P, C.POINTER(TraversalCounts)],
     "cn_renderer_set_profiling": [_P, C.c_int],
     "cn_renderer_kernel_times": [_P, C.POINTER(KernelTimes)],
+    "cn_calibrate_valu": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
     "cn_trace": [_P, _P, _FP, C.c_uint32, C.c_uint32, C.c_uint32],
     "cn_trace_aov": [_P, _P, _FP, _FP, _FP, _FP, C.c_uint32, C.c_uint32, C.c_uint32],
     "cn_first_hits": [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _UP, _UP, _FP],

@@ -2,12 +2,13 @@
 
 The stage kernels' launch shape — the persistent trace grid, the
 shade_surface workgroup variant (1024/512/256 lanes with the BSDF tables in
-LDS, or 256 lanes reading them through the caches) and the shading grid
-caps — does not change any result: every shape renders bitwise the same
+LDS, or 256 lanes reading them through the caches), the shading grid caps
+and the BVH width the traversal walks — does not change any result: every shape renders bitwise the same
 film. It does change speed, and the best shape depends on the scene (BVH
 depth, material mix) and on how many paths a wave holds. ``tune`` times a
 candidate grid on the caller's own workload and leaves the renderer on the
-fastest shape. The reference has no counterpart (its dispatch sizes come
+fastest shape; ``tune_capacity`` picks the path-pool (and so queue)
+capacity the same way. The reference has no counterpart (its dispatch sizes come
 from its queue headers, wavefront.rs:292-356).
 """
 
@@ -30,16 +31,18 @@ SHADE_VARIANTS = {
 }
 
 
-def candidate_shapes(compute_units: int, trace_per_cu=(8, 16, 32), variants=tuple(SHADE_VARIANTS)):
-    """The default search grid: trace workgroups per CU x shade variants,
-    with each shade variant's grid capped at 2048 lanes per CU."""
-    for tpc, variant in itertools.product(trace_per_cu, variants):
+def candidate_shapes(compute_units: int, trace_per_cu=(16, 32, 64), variants=tuple(SHADE_VARIANTS), widths=(4, 8)):
+    """The default search grid: trace workgroups per CU x shade variants x
+    BVH width (the 4- or 8-wide tree a scene carries), with each shade
+    variant's grid capped at 2048 lanes per CU (4096 for the 256-lane ones)."""
+    for tpc, variant, width in itertools.product(trace_per_cu, variants, widths):
         block = SHADE_VARIANTS[variant]
         yield {
             "trace_blocks": compute_units * tpc,
             "shade_variant": variant,
-            "shade_blocks": compute_units * (2048 // block),
+            "shade_blocks": compute_units * ((4096 if block == 256 else 2048) // block),
             "light_blocks": compute_units * 4,
+            "bvh_width": width,
         }
 
 
@@ -80,3 +83,28 @@ def tune(renderer: Renderer, scene: Scene, width: int, height: int, samples: int
     renderer.shape = best
     return {"shape": best, "seconds": best_time,
             "timings": [{"seconds": t, **shape} for t, shape in timings]}
+
+
+def tune_capacity(ctx, scene: Scene, width: int, height: int, samples: int, max_bounces: int,
+                  capacities=(1 << 24, 1 << 25, 1 << 26, 1 << 27), shape=None, repeats: int = 2) -> dict:
+    """The path-pool capacity — and with it every stage queue's capacity,
+    which is sized from the pool — that renders ``samples`` samples of a
+    width x height view of ``scene`` fastest, each candidate on a renderer
+    of its own in ``shape``. Returns the capacity and the timings."""
+    from .render import Renderer
+
+    timings = []
+    for capacity in capacities:
+        r = Renderer(ctx, capacity=capacity, max_bounces=max_bounces)
+        if shape:
+            r.shape = shape
+        film = Film(ctx, width, height)
+        r.reserve(min(samples, r.samples_per_wave(width, height)) * width * height, aov=True)
+        try:
+            timings.append((time_shape(r, scene, film, samples, repeats), capacity))
+        finally:
+            film.close()
+            r.close()
+    best_time, best = min(timings, key=lambda t: t[0])
+    return {"capacity": best, "seconds": best_time,
+            "timings": [{"seconds": t, "capacity": c} for t, c in timings]}

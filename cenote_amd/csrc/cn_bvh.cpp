@@ -176,7 +176,117 @@ struct Collapser {
     }
 };
 
+// The 8-wide collapse: the same greedy opening (the internal child of
+// largest surface area, until eight children or only leaves remain), each
+// node written straight to its quantized form.
+struct Collapser8 {
+    const std::vector<BNode>& bnodes;
+    std::vector<BvhNode8Q>& out;
+    float pad;
+    uint32_t max_depth = 0;
+
+    uint32_t encode(uint32_t b, uint32_t depth) {
+        const BNode& n = bnodes[b];
+        if (n.count > 0) return leaf_encode(n.first, n.count);
+        return collapse(b, depth);
+    }
+
+    uint32_t collapse(uint32_t b, uint32_t depth) {
+        max_depth = std::max(max_depth, depth);
+        const uint32_t index = uint32_t(out.size());
+        out.emplace_back();
+        uint32_t kids[8];
+        int count = 0;
+        const BNode& root = bnodes[b];
+        if (root.count > 0) {
+            kids[count++] = b;
+        } else {
+            kids[count++] = root.left;
+            kids[count++] = root.right;
+            while (count < 8) {
+                int open = -1;
+                float best = -1.0f;
+                for (int k = 0; k < count; ++k) {
+                    const BNode& c = bnodes[kids[k]];
+                    if (c.count == 0 && c.box.area() > best) {
+                        best = c.box.area();
+                        open = k;
+                    }
+                }
+                if (open < 0) break;
+                const BNode& o = bnodes[kids[open]];
+                kids[open] = o.left;
+                kids[count++] = o.right;
+            }
+        }
+        uint32_t encoded[8];
+        Box boxes[8];
+        for (int k = 0; k < count; ++k) {
+            encoded[k] = encode(kids[k], depth + 1);
+            boxes[k] = bnodes[kids[k]].box;
+            for (int a = 0; a < 3; ++a) {
+                boxes[k].lo[a] -= pad;
+                boxes[k].hi[a] += pad;
+            }
+        }
+        out[index] = quantize_wide(boxes, encoded, count, depth);
+        return index;
+    }
+
+    // One node's grid and byte planes, as quantize_node does for 4 children.
+    static BvhNode8Q quantize_wide(const Box* boxes, const uint32_t* child, int count, uint32_t depth) {
+        BvhNode8Q q{};
+        uint32_t* planes_lo[3] = {q.lo_x, q.lo_y, q.lo_z};
+        uint32_t* planes_hi[3] = {q.hi_x, q.hi_y, q.hi_z};
+        float origin[3];
+        for (int k = 0; k < 3; ++k) {
+            double a = INFINITY, b = -INFINITY;
+            for (int c = 0; c < count; ++c) {
+                a = std::min(a, double(boxes[c].lo[k]));
+                b = std::max(b, double(boxes[c].hi[k]));
+            }
+            if (!(a <= b)) a = b = 0.0;
+            origin[k] = float(a);
+            const double extent = b - double(origin[k]);
+            int e = -126;
+            if (extent > 0.0) {
+                std::frexp(extent / 255.0, &e);
+                e = std::max(e, -126);
+                while (std::ldexp(255.0, e) < extent) ++e;
+                while (e > -126 && std::ldexp(255.0, e - 1) >= extent) --e;
+            }
+            const double step = std::ldexp(1.0, e);
+            for (int c = 0; c < count; ++c) {
+                const uint32_t ql = uint32_t(std::floor((double(boxes[c].lo[k]) - double(origin[k])) / step));
+                const uint32_t qh = uint32_t(std::ceil((double(boxes[c].hi[k]) - double(origin[k])) / step));
+                planes_lo[k][c / 4] |= std::min(ql, 255u) << (8 * (c % 4));
+                planes_hi[k][c / 4] |= std::min(qh, 255u) << (8 * (c % 4));
+            }
+            q.exps |= uint32_t(e + 127) << (8 * k);
+        }
+        q.ox = origin[0];
+        q.oy = origin[1];
+        q.oz = origin[2];
+        for (int c = 0; c < 8; ++c) q.child[c] = c < count ? child[c] : CHILD_EMPTY;
+        q.depth = depth;
+        return q;
+    }
+};
+
 }  // namespace
+
+bool collapse_to_bvh8(const std::vector<BinaryNode>& bnodes, float max_abs, uint32_t max_stack, Bvh4* out) {
+    Collapser8 col{bnodes, out->nodes8, 4e-6f * std::fmax(max_abs, 1e-3f)};
+    out->nodes8.clear();
+    out->nodes8.reserve(bnodes.size() / 4 + 1);
+    col.collapse(0, 1);
+    out->max_depth8 = col.max_depth;
+    if (7 * col.max_depth > max_stack) {  // too deep for the stack at 7 pushes per level: 4-wide only
+        out->nodes8.clear();
+        return false;
+    }
+    return true;
+}
 
 bool collapse_to_bvh4(const std::vector<BinaryNode>& bnodes, float max_abs, uint32_t max_stack, Bvh4* out,
                       std::string* error) {
@@ -230,6 +340,7 @@ bool build_bvh4(const std::vector<WorldTriangle>& input, uint32_t max_stack, Bvh
     builder.build(0, 0, n, 0);
     bnodes.resize(builder.next.load());
     if (!collapse_to_bvh4(bnodes, max_abs, max_stack, out, error)) return false;
+    collapse_to_bvh8(bnodes, max_abs, max_stack, out);
 
     out->tris.resize(n);
     for (uint32_t i = 0; i < n; ++i) {

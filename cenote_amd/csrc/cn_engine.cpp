@@ -128,7 +128,8 @@ int ensure_slots(cn_renderer* r, size_t slots, bool aov) {
     return CN_OK;
 }
 
-int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena) {
+int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena,
+                 hipEvent_t accumulate_after) {
     if (jobs.empty()) return CN_OK;
     if (jobs.size() > MAX_JOBS) return fail(CN_E_INVALID, "a wave holds at most %u jobs", MAX_JOBS);
     for (HostJob& j : jobs) {
@@ -201,7 +202,8 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
     // With graphs, every wave reads its job table from one fixed device
     // table (stream order keeps a wave's table until its kernels are done),
     // so a captured launch sequence replays for any wave of the same shape.
-    const bool graph = r->graphs && !r->profiler && !r->counting;
+    // (A wave that waits on another stream's event is launched directly.)
+    const bool graph = r->graphs && !r->profiler && !r->counting && !accumulate_after;
     JobRecord* dev = r->jobs.ptr + (graph ? 0 : size_t(e) * MAX_JOBS);
     uint32_t slot_base = 0, pixel_base = 0;
     for (size_t i = 0; i < jobs.size(); ++i) {
@@ -269,8 +271,10 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
                                      rg.count, b, r->light_sampling, r->counting, r->profiler.get()));
             }
         }
-        if (films)
+        if (films) {
+            if (accumulate_after) CN_HIP(hipStreamWaitEvent(r->stream, accumulate_after, 0));
             CN_HIP(launch_accumulate(r->stream, dev, uint32_t(jobs.size()), tg, uint32_t(pixels), r->profiler.get()));
+        }
         return CN_OK;
     };
 
@@ -283,7 +287,7 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
                                      uint64_t(uintptr_t(tg.radiance)), uint64_t(uintptr_t(pool.origin)),
                                      uint64_t(uintptr_t(pool.stack_overflow)), r->shape.trace_blocks,
                                      r->shape.shade_variant, r->shape.shade_blocks, r->shape.light_blocks,
-                                     r->light_sampling};
+                                     r->shape.bvh_width, r->light_sampling};
         for (const Range& rg : plan) {
             key.insert(key.end(), {jobs[rg.g0].scene->serial, uint64_t(rg.g0), uint64_t(rg.g1), rg.base, rg.count,
                                    rg.rounds});
@@ -371,6 +375,7 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     r->shape.shade_variant = SHADE_256_GLOBAL_SORTED;
     r->shape.shade_blocks = cus * 16;
     r->shape.light_blocks = cus * 4;
+    r->shape.bvh_width = 4;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * TRAVERSAL_BLOCK * STACK_SPILL));
     CN_HIP(r->traversal.alloc(4));
@@ -415,6 +420,7 @@ int cn_renderer_get_shape(cn_renderer* r, cn_launch_shape* out) {
     out->shade_variant = r->shape.shade_variant;
     out->shade_blocks = r->shape.shade_blocks;
     out->light_blocks = r->shape.light_blocks;
+    out->bvh_width = r->shape.bvh_width;
     return CN_OK;
 }
 
@@ -425,6 +431,8 @@ int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
         return fail(CN_E_INVALID, "launch shape block counts must lie in 1..65536");
     if (shape->shade_variant >= SHADE_VARIANTS)
         return fail(CN_E_INVALID, "unknown shade variant %u", shape->shade_variant);
+    if (shape->bvh_width != 4 && shape->bvh_width != 8)
+        return fail(CN_E_INVALID, "BVH width %u: 4 or 8", shape->bvh_width);
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
     // The traversal stack's spill area is per lane of the trace grid.
@@ -438,6 +446,7 @@ int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
     r->shape.shade_variant = shape->shade_variant;
     r->shape.shade_blocks = shape->shade_blocks;
     r->shape.light_blocks = shape->light_blocks;
+    r->shape.bvh_width = shape->bvh_width;
     return CN_OK;
 }
 

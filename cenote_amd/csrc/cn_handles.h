@@ -38,6 +38,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), count(o.count) {
+        o.ptr = nullptr;
+        o.count = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            ptr = o.ptr;
+            count = o.count;
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { reset(); }
     void reset() {
         if (ptr) (void)hipFree(ptr);
@@ -95,6 +109,8 @@ struct cn_scene {
     cn_camera camera{};
     cn::DevBuf<cn::BvhNode4Q> nodes;        // what the traversal reads
     std::vector<cn::BvhNode4> host_nodes;    // the full-precision tree (cn_scene_bvh)
+    cn::DevBuf<cn::BvhNode8Q> nodes8;       // the 8-wide tree over the same leaves (may be empty)
+    std::vector<cn::BvhNode8Q> host_nodes8;
     cn::DevBuf<cn::BvhTriangle> tris;
     cn::DevBuf<cn::InstanceRecord> instances;
     cn::DevBuf<cn_material> materials;
@@ -106,6 +122,17 @@ struct cn_scene {
     cn::DevBuf<uint32_t> env_marginal_guide, env_conditional_guide;
     std::vector<cn_light_record> host_lights;
     cn::EnvironmentTables host_env;
+    uint32_t env_width = 0, env_height = 0;
+    // What an incremental update (cn_scene_update) rebuilds from: the light
+    // list's inputs and total power, and the placement it checks edits
+    // against (per-instance records and mesh, per-mesh vertex and triangle
+    // counts).
+    std::vector<cn::EmissiveTriangle> host_emitters;
+    std::vector<cn_delta_light> host_deltas;
+    double light_power = 0.0;
+    std::vector<cn::InstanceRecord> host_records;
+    std::vector<uint32_t> host_instance_mesh;
+    std::vector<uint32_t> host_mesh_shape;
     cn_scene_info info{};
     cn::SceneView view{};
 };
@@ -152,8 +179,12 @@ struct WaveArena {
 // Enqueue one wave of jobs on the renderer's stream (no host sync unless a
 // buffer must grow or a job-table ring entry is still in flight). With an
 // arena the wave runs in it and nothing is allocated; a wave of more slots
-// than the arena holds fails with CN_E_INVALID.
-int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena = nullptr);
+// than the arena holds fails with CN_E_INVALID. With `accumulate_after`,
+// the wave's accumulate waits for that event (a wave on another stream
+// that folds into the same films), so films still see their samples in
+// order while the waves' tracing overlaps.
+int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena = nullptr,
+                 hipEvent_t accumulate_after = nullptr);
 
 // A film job for `samples` more samples of `camera` on `s`, continuing at
 // the film's sample count.

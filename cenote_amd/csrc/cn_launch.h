@@ -34,6 +34,7 @@ struct LaunchShape {
     uint32_t shade_variant;  // a ShadeVariant
     uint32_t shade_blocks;   // most blocks (of the variant's size) for shade_surface
     uint32_t light_blocks;   // blocks of 256 lanes for shade_miss
+    uint32_t bvh_width;      // 4 or 8: the tree the traversal kernels walk
 };
 
 // The film's buffers (render/film.rs:66-127): one wave's sample

@@ -144,6 +144,67 @@ __host__ __device__ __forceinline__ float hash_uniform(uint32_t pixel, uint32_t 
     return unit_float(h);
 }
 
+// Stochastic opacity's Bernoulli trial for one crossing of a
+// fractional-opacity instance. The reference (intersect.slang:64-76) keys
+// its draw on the crossing's rank in the driver's traversal order, which no
+// other traversal reproduces. Here the draw is the hashUniform stream of
+// (pixel, sample, bounce) with the crossing's (instance, primitive) folded
+// in where the rank went: every surface a ray meets makes one independent
+// trial whatever order the BVH is walked in, so the hit is the nearest
+// crossing whose trial commits — the distribution the reference's comment
+// describes ("the nearest crossing that commits wins").
+__host__ __device__ __forceinline__ float crossing_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce,
+                                                           uint32_t instance, uint32_t primitive) {
+    uint32_t h = lowbias32(pixel ^ lowbias32(sample ^ 0x9e3779b9u));
+    h = lowbias32(h ^ (bounce << 8));
+    h = lowbias32(h ^ lowbias32(instance ^ lowbias32(primitive ^ 0x85ebca6bu)));
+    return unit_float(h);
+}
+
+// Fractional-opacity shadows multiply (1 - opacity) per crossing
+// (trace_shadow.slang:52-65). A float product depends on the order of its
+// factors and traversals meet crossings in different orders, so the product
+// is kept as a sum of base-2 logarithms in fixed point (2^-24 units): integer
+// adds, equal in any order. Each instance's log2(1 - opacity) is rounded to
+// that grid once, on the host (log2_transmittance_fixed, cn_prep.cpp), and
+// exp2_fixed maps a sum back. The reference ends a connection once its
+// attenuation falls below 1e-4; here once the sum falls below
+// ATTENUATION_CUTOFF = ceil(log2(1e-4) * 2^24). The sum never rises, so a
+// walk may stop as soon as it is below.
+constexpr int32_t LOG2_FIXED_SHIFT = 24;
+constexpr int32_t LOG2_FIXED_FLOOR = -(1 << 28);  // one crossing of transmittance <= 2^-16
+constexpr int32_t ATTENUATION_CUTOFF = -222930820;
+
+// 2^(L / 2^24) for LOG2_FIXED_FLOOR * 2 <= L <= 0: e^x on [0, ln 2) by its
+// Taylor polynomial to x^12 (truncation below 1e-11 relative) in double,
+// Horner form, rounded once to float, times an exact power of two. Only
+// IEEE adds and multiplies: the same bits on every processor built without
+// contraction.
+__host__ __device__ inline float exp2_fixed(int32_t L) {
+    const int32_t n = L >> LOG2_FIXED_SHIFT;  // floor(L / 2^24)
+    const int32_t f = L - n * (1 << LOG2_FIXED_SHIFT);
+    const double x = double(f) * (0.6931471805599453 / 16777216.0);
+    double p = 2.08767569878681e-09;  // 1/12!
+    p = p * x + 2.505210838544172e-08;
+    p = p * x + 2.755731922398589e-07;
+    p = p * x + 2.7557319223985893e-06;
+    p = p * x + 2.48015873015873e-05;
+    p = p * x + 0.0001984126984126984;
+    p = p * x + 0.001388888888888889;
+    p = p * x + 0.008333333333333333;
+    p = p * x + 0.041666666666666664;
+    p = p * x + 0.16666666666666666;
+    p = p * x + 0.5;
+    p = p * x + 1.0;
+    p = p * x + 1.0;
+    union {
+        uint32_t u;
+        float f;
+    } scale;
+    scale.u = uint32_t(127 + (n < -126 ? -126 : n)) << 23;  // 2^n, exact
+    return float(p) * scale.f;
+}
+
 struct f2 {
     float x, y;
 };

@@ -64,6 +64,14 @@ uint32_t material_class(const cn_material& m) {
            (m.coat_weight > 0.0f ? 8u : 0u) | (m.fuzz_weight > 0.0f ? 16u : 0u);
 }
 
+int32_t log2_transmittance_fixed(float opacity) {
+    const double transmittance = 1.0 - double(opacity);
+    if (!(transmittance > 0.0)) return LOG2_FIXED_FLOOR;
+    if (transmittance >= 1.0) return 0;
+    const double units = std::nearbyint(std::log2(transmittance) * double(1 << LOG2_FIXED_SHIFT));
+    return units < double(LOG2_FIXED_FLOOR) ? LOG2_FIXED_FLOOR : int32_t(units);
+}
+
 std::vector<cn_light_record> build_light_table(const std::vector<EmissiveTriangle>& tris,
                                                const cn_delta_light* deltas, uint32_t delta_count,
                                                double* total_power) {

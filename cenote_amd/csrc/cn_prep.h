@@ -29,6 +29,12 @@ float luminance(v3 c);
 // through the closure.
 uint32_t material_class(const cn_material& m);
 
+// A fractional-opacity instance's log2(1 - opacity) in 2^-24 units, rounded
+// to nearest and floored at LOG2_FIXED_FLOOR (one crossing that dark
+// already ends a connection): the per-crossing term of the shadow
+// attenuation's order-free product (exp2_fixed, cn_math.h).
+int32_t log2_transmittance_fixed(float opacity);
+
 // lights.rs:175-224, :229-231. Returns the records; *total_power is the
 // list's total selection weight.
 std::vector<cn_light_record> build_light_table(const std::vector<EmissiveTriangle>& tris,

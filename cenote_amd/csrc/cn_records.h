@@ -99,6 +99,20 @@ struct alignas(16) BvhNode4Q {
 };
 static_assert(sizeof(BvhNode4Q) == 64, "half a 128-byte line per node");
 
+// The 8-wide traversal node: eight children on the same kind of per-node
+// grid, one 128-byte line. Byte c of each plane pair (lo_x[0] holds
+// children 0..3, lo_x[1] children 4..7) is child c's plane in grid steps;
+// the traversal reads the first 96 bytes (six dwordx4 fetches). Collapsed
+// from the same binary tree as the 4-wide nodes, over the same leaves.
+struct alignas(16) BvhNode8Q {
+    float ox, oy, oz;
+    uint32_t exps;
+    uint32_t lo_x[2], hi_x[2], lo_y[2], hi_y[2], lo_z[2], hi_z[2];
+    uint32_t child[8];
+    uint32_t depth, pad[7];
+};
+static_assert(sizeof(BvhNode8Q) == 128, "one 128-byte line per 8-wide node");
+
 constexpr uint32_t CHILD_EMPTY = 0xffffffffu;
 constexpr uint32_t CHILD_LEAF = 0x80000000u;
 constexpr uint32_t LEAF_MAX = 16;
@@ -139,7 +153,9 @@ struct alignas(16) InstanceRecord {
     uint32_t vertex_base;
     uint32_t index_base;  // in triangles
     uint32_t flags;       // bits 0..7 = visibility mask, bit 8 = opaque
-    uint32_t pad[4];      // pad[0]: the material class (material_class), for material-sorted shading
+    uint32_t material_class;     // which lobes the closure has (material-sorted shading)
+    int32_t log2_transmittance;  // log2(1 - opacity) in 2^-24 units (log2_transmittance_fixed)
+    uint32_t pad[2];
 };
 static_assert(sizeof(InstanceRecord) == 128, "two dwordx4 lines per instance");
 
@@ -149,6 +165,7 @@ static_assert(sizeof(cn_light_record) == 80, "lights.rs:115");
 // Everything the stage kernels read of a scene, passed by value.
 struct SceneView {
     const BvhNode4Q* nodes;
+    const BvhNode8Q* nodes8;  // the 8-wide tree over the same leaves (null when the scene has none)
     const BvhTriangle* tris;
     const InstanceRecord* instances;
     const cn_material* materials;

@@ -7,10 +7,14 @@
 // Per step: admit queued requests (FIFO) while their films fit the budget;
 // build one wave — every admitted request that has no sample yet gets one
 // first (the first frame), then every unfinished request one more, then
-// leftover pool room goes to extra samples in admission order; enqueue it;
-// retire the oldest in-flight wave once two are in flight. A request's film
-// sees its samples in index order, so its result is bitwise the same
-// whether it is served alone or in any mix.
+// leftover pool room goes to extra samples in admission order; enqueue it
+// on the next of LANES lanes — renderers of the server's own, each with its
+// stream and path pool — so that up to LANES waves trace at once and one
+// wave's sparse deep bounces overlap the next waves' full ones; retire the
+// oldest wave once LANES are in flight. Every wave's accumulate waits for
+// the previous wave's (an event across lanes), so a request's film sees its
+// samples in index order and its result is bitwise the same whether it is
+// served alone or in any mix.
 
 #include <hip/hip_runtime.h>
 
@@ -29,7 +33,7 @@ using namespace cn;
 namespace {
 
 constexpr uint32_t NOISE_BLOCKS = 64;  // partial sums per noise estimate
-constexpr uint32_t IN_FLIGHT = 2;      // waves enqueued ahead of the host
+constexpr uint32_t LANES = 3;          // waves tracing at once, one per lane
 
 size_t film_bytes(uint32_t width, uint32_t height, bool noise) {
     // beauty, albedo, normal sums (16 B each) + depth (4 B) [+ moments (4 B)]
@@ -65,7 +69,9 @@ struct InFlight {
 }  // namespace
 
 struct cn_server {
-    cn_renderer* r = nullptr;
+    cn_renderer* r = nullptr;           // the caller's renderer: depth, light sampling, launch shape
+    std::vector<cn_renderer*> lanes;    // the server's own renderers, one wave in flight each
+    uint32_t lane_next = 0;
     uint64_t budget = 0;
     uint32_t wave_paths = 0;
     std::chrono::steady_clock::time_point t0;
@@ -80,7 +86,7 @@ struct cn_server {
     uint64_t next_id = 1;
     std::deque<InFlight> inflight;
     std::vector<hipEvent_t> free_events;
-    // Noise partials: a ring of IN_FLIGHT + 1 entries of MAX_JOBS estimates.
+    // Noise partials: a ring of LANES + 1 entries of MAX_JOBS estimates.
     DevBuf<double> partials;
     double* host_partials = nullptr;  // pinned
     uint32_t ring_next = 0;
@@ -94,7 +100,7 @@ struct cn_server {
 
 namespace {
 
-constexpr uint32_t NOISE_RING = IN_FLIGHT + 1;
+constexpr uint32_t NOISE_RING = LANES + 1;
 
 int admit(cn_server* sv) {
     while (!sv->queue.empty() && sv->active.size() < MAX_JOBS) {
@@ -179,7 +185,7 @@ int retire_one(cn_server* sv) {
 }
 
 int enqueue(cn_server* sv) {
-    cn_renderer* r = sv->r;
+    cn_renderer* r = sv->lanes[sv->lane_next];
     std::vector<HostJob> jobs;
     std::vector<uint64_t> ids;
     std::vector<uint32_t> counts;
@@ -217,7 +223,9 @@ int enqueue(cn_server* sv) {
         counts[k] += uint32_t(extra);
         room -= extra * px;
     }
-    if (int e = enqueue_wave(r, jobs, true)) return e;
+    hipEvent_t previous = sv->inflight.empty() ? nullptr : sv->inflight.back().event;
+    if (int e = enqueue_wave(r, jobs, true, nullptr, previous)) return e;
+    sv->lane_next = (sv->lane_next + 1) % LANES;
     InFlight w;
     w.ring = sv->ring_next;
     sv->ring_next = (sv->ring_next + 1) % NOISE_RING;
@@ -268,9 +276,29 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
     sv->budget = hbm_budget_bytes;
     sv->wave_paths = std::min(wave_paths == 0 ? (1u << 24) : wave_paths, renderer->capacity);
     sv->t0 = std::chrono::steady_clock::now();
-    CN_HIP(sv->partials.alloc(size_t(NOISE_RING) * MAX_JOBS * NOISE_BLOCKS));
-    CN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sv->host_partials),
-                         sizeof(double) * NOISE_RING * MAX_JOBS * NOISE_BLOCKS, hipHostMallocDefault));
+    // The lanes copy the caller's renderer's depth, light sampling and launch
+    // shape; each holds a pool of wave_paths paths, allocated here.
+    auto lanes_up = [&]() -> int {
+        cn_launch_shape shape;
+        if (int e = cn_renderer_get_shape(renderer, &shape)) return e;
+        for (uint32_t k = 0; k < LANES; ++k) {
+            cn_renderer* lane = nullptr;
+            if (int e = cn_renderer_create(renderer->ctx, sv->wave_paths, renderer->max_bounces,
+                                           renderer->light_sampling, &lane))
+                return e;
+            sv->lanes.push_back(lane);
+            if (int e = cn_renderer_set_shape(lane, &shape)) return e;
+            if (int e = cn_renderer_reserve(lane, sv->wave_paths, 1)) return e;
+        }
+        CN_HIP(sv->partials.alloc(size_t(NOISE_RING) * MAX_JOBS * NOISE_BLOCKS));
+        CN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sv->host_partials),
+                             sizeof(double) * NOISE_RING * MAX_JOBS * NOISE_BLOCKS, hipHostMallocDefault));
+        return CN_OK;
+    };
+    if (int e = lanes_up()) {
+        for (cn_renderer* lane : sv->lanes) cn_renderer_destroy(lane);
+        return e;
+    }
     *out = sv.release();
     return CN_OK;
 }
@@ -278,12 +306,13 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
 int cn_server_destroy(cn_server* sv) {
     if (!sv) return fail(CN_E_INVALID, "cn_server_destroy: null server");
     (void)hipSetDevice(sv->r->ctx->device);
-    (void)hipStreamSynchronize(sv->r->stream);
+    for (cn_renderer* lane : sv->lanes) (void)hipStreamSynchronize(lane->stream);
     for (auto& w : sv->inflight) (void)hipEventDestroy(w.event);
     for (hipEvent_t e : sv->free_events) (void)hipEventDestroy(e);
     for (auto& [id, q] : sv->requests) delete q.film;
     for (auto& [size, f] : sv->spare) delete f;
     if (sv->host_partials) (void)hipHostFree(sv->host_partials);
+    for (cn_renderer* lane : sv->lanes) cn_renderer_destroy(lane);
     delete sv;
     return CN_OK;
 }
@@ -327,9 +356,9 @@ int cn_server_step(cn_server* sv, uint32_t* busy) {
     const size_t before = sv->inflight.size();
     if (int e = enqueue(sv)) return e;
     const bool enqueued = sv->inflight.size() > before;
-    // Keep at most IN_FLIGHT - 1 waves queued behind the running one; when
-    // there was nothing new to enqueue, retire so finishing requests report.
-    if (sv->inflight.size() >= IN_FLIGHT || (!enqueued && !sv->inflight.empty())) {
+    // Keep at most LANES waves in flight; when there was nothing new to
+    // enqueue, retire so finishing requests report.
+    if (sv->inflight.size() >= LANES || (!enqueued && !sv->inflight.empty())) {
         if (int e = retire_one(sv)) return e;
     }
     if (busy) *busy = uint32_t(sv->queue.size() + sv->active.size() + sv->inflight.size());

@@ -194,12 +194,18 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
 // Two half-empty launches per bounce become one fuller one, which is what
 // the deep, sparse bounces need.
 
+// The 8-wide walk holds eight keys and refs through its sorting network:
+// at eight waves per SIMD (64 VGPRs) it spills, so it runs at fewer.
+#ifndef TRACE8_WAVES
+#define TRACE8_WAVES 7
+#endif
+
 // COUNT: accumulate node and triangle fetches (the counting pass); the
 // timed form keeps no counters. Eight waves per SIMD (64 VGPRs; LDS 8 x 16
 // KB per CU) fit without scratch: traversal waits on memory, so every
 // extra wave hides latency.
-template <bool COUNT>
-__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+template <bool COUNT, bool W8>
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(W8 ? TRACE8_WAVES : 8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                            uint32_t range_count, uint32_t bounce) {
     __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
     // Bounce 0 traces the camera paths: item i is path i.
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
             const uint2 key = p.key[path];
-            const TraceHit h = trace_closest(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
+            const TraceHit h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
             cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
@@ -234,9 +240,9 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             // (tmax / 1.001 for a triangle target) lies in front of the
             // target, so it occludes without finding the closest hit.
             const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
-            const float att = trace_visibility(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[j], t_early, st,
+            const float att = trace_visibility<W8>(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[j], t_early, st,
                                                shadow_nodes, shadow_tris);
-            if (att >= 1e-4f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
+            if (att > 0.0f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
         }
         if (first < n_ray) {  // uniform per block: some wave of this block holds rays
             const uint32_t split = chunk_split(first);
@@ -742,6 +748,7 @@ __global__ void k_sample_sequence(uint32_t pixel, uint32_t dimension, uint32_t c
 }
 
 // The bounce-0 closest hit per pixel, for the first-hit identity check.
+template <bool W8>
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_first_hits(SceneView s, CameraView cam, PathPool p, uint32_t width,
                                                       uint32_t height, uint32_t sample, uint32_t* inst_out,
                                                       uint32_t* prim_out, float* bary_out) {
@@ -756,7 +763,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) void k_first_hits(SceneView s, Cam
         v3 o, d;
         float cf;
         camera_ray(cam, width, height, pixel, sample, &o, &d, &cf);
-        const TraceHit h = trace_closest(s, o, d, 1e30f, MASK_CAMERA, st, pixel, sample, 0, nodes, tris);
+        const TraceHit h = trace_closest<W8>(s, o, d, 1e30f, MASK_CAMERA, st, pixel, sample, 0, nodes, tris);
         inst_out[pixel] = h.hit ? h.inst : 0xffffffffu;
         prim_out[pixel] = h.hit ? h.prim : 0xffffffffu;
         bary_out[2 * pixel] = h.hit ? h.b1 : 0.0f;
@@ -822,12 +829,18 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     // Bounce b's rays and bounce b-1's shadow rays share one launch; the
     // last bounce shades terminally and leaves no shadow rays behind.
     if (prof) prof->begin(K_TRACE);
-    if (counting)
-        hipLaunchKernelGGL(k_trace<true>, dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, range_count,
-                           bounce);
-    else
-        hipLaunchKernelGGL(k_trace<false>, dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, range_count,
-                           bounce);
+    const bool wide = shape.bvh_width == 8 && s.nodes8 != nullptr;
+#define CN_TRACE(COUNT, W8)                                                                                          \
+    hipLaunchKernelGGL((k_trace<COUNT, W8>), dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, \
+                       range_count, bounce)
+    if (counting) {
+        if (wide) CN_TRACE(true, true);
+        else CN_TRACE(true, false);
+    } else {
+        if (wide) CN_TRACE(false, true);
+        else CN_TRACE(false, false);
+    }
+#undef CN_TRACE
     if (prof) prof->end(K_TRACE);
     if (prof) prof->begin(K_SHADE_MISS);
     hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
@@ -885,8 +898,12 @@ hipError_t launch_sample_sequence(hipStream_t stream, uint32_t pixel, uint32_t d
 hipError_t launch_first_hits(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const CameraView& cam,
                              const PathPool& p, uint32_t width, uint32_t height, uint32_t sample, uint32_t* inst,
                              uint32_t* prim, float* bary) {
-    hipLaunchKernelGGL(k_first_hits, dim3(shape.trace_blocks), dim3(TRAVERSAL_BLOCK), 0, stream, s, cam, p, width, height, sample,
-                       inst, prim, bary);
+    if (shape.bvh_width == 8 && s.nodes8 != nullptr)
+        hipLaunchKernelGGL(k_first_hits<true>, dim3(shape.trace_blocks), dim3(TRAVERSAL_BLOCK), 0, stream, s, cam, p,
+                           width, height, sample, inst, prim, bary);
+    else
+        hipLaunchKernelGGL(k_first_hits<false>, dim3(shape.trace_blocks), dim3(TRAVERSAL_BLOCK), 0, stream, s, cam, p,
+                           width, height, sample, inst, prim, bary);
     return hipGetLastError();
 }
 

@@ -16,8 +16,6 @@
 
 namespace cn {
 
-constexpr uint32_t TRANSPARENT_CAP = 8;     // intersect.slang:23
-constexpr uint32_t ATTENUATION_CROSSINGS = 16;
 constexpr uint32_t NODE_DONE = CHILD_EMPTY;  // never pushed: empty slots fail every box test
 
 // The traversal stack: the first STACK_LDS entries of each lane live in LDS
@@ -64,6 +62,23 @@ struct Stack {
         if (nh > 3) push(c3);
         if (nh > 2) push(c2);
         if (nh > 1) push(c1);
+    }
+    // The 8-wide form: ref[1..7] pushed farthest first, the ones past the
+    // hit count landing above the new top.
+    __device__ __forceinline__ void push_sorted8(const uint32_t (&ref)[8], uint32_t nh) {
+        if (sp + 7 <= STACK_LDS) {
+            uint32_t at = sp;
+#pragma unroll
+            for (int k = 7; k >= 1; --k) {
+                lds[at * TRAVERSAL_BLOCK + threadIdx.x] = ref[k];
+                at += uint32_t(k) < nh ? 1u : 0u;
+            }
+            sp = at;
+            return;
+        }
+#pragma unroll
+        for (int k = 7; k >= 1; --k)
+            if (uint32_t(k) < nh) push(ref[k]);
     }
     __device__ __forceinline__ uint32_t pop_or_done() {
         if (sp == 0) return NODE_DONE;
@@ -197,8 +212,8 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 // (the same values the min/max form gives; a child's low byte never
 // exceeds its high one). The boxes' padding (far above these roundings)
 // keeps the test conservative.
-__device__ __forceinline__ uint32_t visit_node(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
-                                               float bound, Stack& st) {
+__device__ __forceinline__ uint32_t visit(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
+                                          float bound, Stack& st) {
     const BvhNode4Q n = nodes[index];
     const float sx = __uint_as_float((n.exps & 0xffu) << 23) * r.ix;
     const float sy = __uint_as_float(((n.exps >> 8) & 0xffu) << 23) * r.iy;
@@ -234,17 +249,77 @@ __device__ __forceinline__ uint32_t visit_node(const BvhNode4Q* __restrict__ nod
     return ref[0];
 }
 
+// The same box test over an 8-wide node (six dwordx4 fetches), its hit
+// children ordered by entry distance with Batcher's odd-even merge network
+// (19 compare-exchanges in six rounds; misses keyed +inf sort last). Ties
+// may leave slot order, which changes only the visiting order: the closest
+// hit, visibility and transmittance do not depend on it.
+__device__ __forceinline__ uint32_t visit(const BvhNode8Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
+                                          float bound, Stack& st) {
+    const uint4* line = reinterpret_cast<const uint4*>(nodes + index);
+    const uint4 h = line[0];
+    const uint4 px = line[1];  // lo_x[0], lo_x[1], hi_x[0], hi_x[1]
+    const uint4 py = line[2];
+    const uint4 pz = line[3];
+    const uint4 ca = line[4];
+    const uint4 cb = line[5];
+    const float sx = __uint_as_float((h.w & 0xffu) << 23) * r.ix;
+    const float sy = __uint_as_float(((h.w >> 8) & 0xffu) << 23) * r.iy;
+    const float sz = __uint_as_float(((h.w >> 16) & 0xffu) << 23) * r.iz;
+    const float bx0 = (__uint_as_float(h.x) - r.x) * r.ix;
+    const float by0 = (__uint_as_float(h.y) - r.y) * r.iy;
+    const float bz0 = (__uint_as_float(h.z) - r.z) * r.iz;
+    const uint32_t nx[2] = {r.nx ? px.z : px.x, r.nx ? px.w : px.y}, fx[2] = {r.nx ? px.x : px.z, r.nx ? px.y : px.w};
+    const uint32_t ny[2] = {r.ny ? py.z : py.x, r.ny ? py.w : py.y}, fy[2] = {r.ny ? py.x : py.z, r.ny ? py.y : py.w};
+    const uint32_t nz[2] = {r.nz ? pz.z : pz.x, r.nz ? pz.w : pz.y}, fz[2] = {r.nz ? pz.x : pz.z, r.nz ? pz.y : pz.w};
+    const uint32_t ch[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    float key[8];
+    uint32_t ref[8];
+    uint32_t nh = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int w = c >> 2, b = c & 3;
+        const float ax = fmaf(plane(nx[w], b), sx, bx0), bx = fmaf(plane(fx[w], b), sx, bx0);
+        const float ay = fmaf(plane(ny[w], b), sy, by0), by = fmaf(plane(fy[w], b), sy, by0);
+        const float az = fmaf(plane(nz[w], b), sz, bz0), bz = fmaf(plane(fz[w], b), sz, bz0);
+        const float t0 = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+        const float t1 = fminf(fminf(bx, by), fminf(bz, bound)) * 1.0000004f;
+        const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
+        key[c] = hit ? t0 : __int_as_float(0x7f800000);
+        ref[c] = hit ? ch[c] : NODE_DONE;
+        nh += hit ? 1u : 0u;
+    }
+    if (nh == 0) return st.pop_or_done();
+#define CN_CE(a, b) order_pair(key[a], ref[a], key[b], ref[b])
+    CN_CE(0, 1); CN_CE(2, 3); CN_CE(4, 5); CN_CE(6, 7);
+    CN_CE(0, 2); CN_CE(1, 3); CN_CE(4, 6); CN_CE(5, 7);
+    CN_CE(1, 2); CN_CE(5, 6);
+    CN_CE(0, 4); CN_CE(1, 5); CN_CE(2, 6); CN_CE(3, 7);
+    CN_CE(2, 4); CN_CE(3, 5);
+    CN_CE(1, 2); CN_CE(3, 4); CN_CE(5, 6);
+#undef CN_CE
+    st.push_sorted8(ref, nh);
+    return ref[0];
+}
+
+// The tree a traversal walks: the 4-wide nodes, or the 8-wide ones.
+template <bool W8>
+__device__ __forceinline__ auto tree_of(const SceneView& s) {
+    if constexpr (W8) return s.nodes8;
+    else return s.nodes;
+}
+
 // The speculative while-while walk over [0, bound]. `leaf(ref)` tests one
 // leaf's triangles (it may shrink `bound`) and returns true to end the ray.
-template <typename Leaf>
-__device__ __forceinline__ void walk(const BvhNode4Q* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
+template <typename Node, typename Leaf>
+__device__ __forceinline__ void walk(const Node* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
                                      uint32_t& nodes_visited, Leaf&& leaf_fn) {
     st.sp = 0;
     uint32_t cur = 0;  // the root is always an internal node
     uint32_t leaf = NODE_DONE;
     while (true) {
         while (!(cur & CHILD_LEAF)) {
-            cur = visit_node(nodes, cur, r, bound, st);
+            cur = visit(nodes, cur, r, bound, st);
             ++nodes_visited;
             if ((cur & CHILD_LEAF) && cur != NODE_DONE && leaf == NODE_DONE) {
                 leaf = cur;
@@ -277,16 +352,18 @@ struct TraceHit {
     uint32_t flags;  // the hit triangle's flags (mask, opacity, material class)
 };
 
-// Closest hit among triangles whose instance mask meets `mask`. Crossings
-// of fractional-opacity instances commit stochastically
-// (intersect.slang:172-188): hashUniform(pixel, sample, bounce, crossing) <
-// opacity, or once TRANSPARENT_CAP crossings have passed. Equal-t ties go
-// to the lower (instance, primitive).
-// The closest-hit leaf test (a crossing of a fractional-opacity instance
-// draws its commit from the pixel's hash stream).
+// Closest hit among triangles whose instance mask meets `mask`. A crossing
+// of a fractional-opacity instance commits when its own Bernoulli trial
+// passes (crossing_uniform(pixel, sample, bounce, instance, primitive) <
+// opacity; intersect.slang:64-76), so the hit is the nearest crossing whose
+// trial commits, in any traversal order. The reference also commits every
+// crossing past its eighth in traversal order (intersect.slang:20-23); with
+// per-surface trials the walk needs no such bound, and without it the
+// result does not depend on the order. Equal-t ties go to the lower
+// (instance, primitive).
 __device__ __forceinline__ void closest_leaf(const SceneView& s, const WRay& w, uint32_t ref, uint32_t mask,
                                              uint32_t pixel, uint32_t sample, uint32_t bounce, TraceHit& best,
-                                             uint32_t& crossings, uint32_t& tris_tested) {
+                                             uint32_t& tris_tested) {
     uint32_t first, count;
     leaf_range(ref, &first, &count);
     for (uint32_t k = 0; k < count; ++k) {
@@ -298,25 +375,22 @@ __device__ __forceinline__ void closest_leaf(const SceneView& s, const WRay& w, 
         if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), best.t, &t, &b1, &b2)) continue;
         const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
         if (best.hit && t == best.t && (inst > best.inst || (inst == best.inst && prim > best.prim))) continue;
-        if (flags & TRI_NON_OPAQUE) {
-            const float opacity = s.materials[inst].opacity;
-            const bool commit = crossings >= TRANSPARENT_CAP || hash_uniform(pixel, sample, bounce, crossings) < opacity;
-            ++crossings;
-            if (!commit) continue;
-        }
+        if ((flags & TRI_NON_OPAQUE) &&
+            !(crossing_uniform(pixel, sample, bounce, inst, prim) < s.materials[inst].opacity))
+            continue;
         best = TraceHit{true, t, inst, prim, b1, b2, flags};
     }
 }
 
+template <bool W8>
 __device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d, float tmax, uint32_t mask,
                                                   Stack& st, uint32_t pixel, uint32_t sample, uint32_t bounce,
                                                   uint32_t& nodes_visited, uint32_t& tris_tested) {
     const SlabRay r = make_slab(o, d);
     const WRay w = make_wray(o, d);
     TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
-    uint32_t crossings = 0;
-    walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
-        closest_leaf(s, w, ref, mask, pixel, sample, bounce, best, crossings, tris_tested);
+    walk(tree_of<W8>(s), r, best.t, st, nodes_visited, [&](uint32_t ref) {
+        closest_leaf(s, w, ref, mask, pixel, sample, bounce, best, tris_tested);
         return false;
     });
     return best;
@@ -348,19 +422,19 @@ __device__ __forceinline__ bool visibility_leaf(const SceneView& s, const WRay& 
     return false;
 }
 
+template <bool W8>
 __device__ inline float shadow_attenuation(const SceneView& s, const SlabRay& r, const WRay& w, float t_end,
                                            uint32_t target_inst, uint32_t target_prim, Stack& st,
                                            uint32_t& nodes_visited);
 
 // A next-event connection's visibility (trace_shadow.slang:41-71): visible
 // iff nothing solid commits within tmax or the committed hit is the target
-// (instance, primitive). Returns the attenuation (0 when occluded).
-// Fractional-opacity crossings in front of the connection's end multiply
-// their transparency in, nearest first — the reference multiplies them in
-// the driver's traversal order, which no test pins; nearest-first is an
-// order any traversal reproduces. Only the first ATTENUATION_CROSSINGS
-// count. A solid non-target hit nearer than `t_early` occludes without
-// looking further.
+// (instance, primitive). Returns the attenuation (0 when occluded or
+// attenuated below 1e-4): the transmittance of every fractional-opacity
+// crossing in front of the connection's end, taken in an order-free form
+// (exp2_fixed of a sum of fixed-point logarithms). A solid non-target hit
+// nearer than `t_early` occludes without looking further.
+template <bool W8>
 __device__ __forceinline__ float trace_visibility(const SceneView& s, v3 o, v3 d, float tmax, uint32_t target_inst,
                                                   uint32_t target_prim, float t_early, Stack& st,
                                                   uint32_t& nodes_visited, uint32_t& tris_tested) {
@@ -368,58 +442,44 @@ __device__ __forceinline__ float trace_visibility(const SceneView& s, v3 o, v3 d
     const WRay w = make_wray(o, d);
     TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
     bool occluded = false, translucent = false;
-    walk(s.nodes, r, best.t, st, nodes_visited, [&](uint32_t ref) {
+    walk(tree_of<W8>(s), r, best.t, st, nodes_visited, [&](uint32_t ref) {
         occluded = visibility_leaf(s, w, ref, target_inst, target_prim, t_early, best, translucent, tris_tested);
         return occluded;
     });
     if (occluded) return 0.0f;
     if (best.hit && !(best.inst == target_inst && best.prim == target_prim)) return 0.0f;
     if (!translucent) return 1.0f;
-    return shadow_attenuation(s, r, w, best.hit ? best.t : tmax, target_inst, target_prim, st, nodes_visited);
+    return shadow_attenuation<W8>(s, r, w, best.hit ? best.t : tmax, target_inst, target_prim, st, nodes_visited);
 }
 
 // The rare path of a visible connection that crossed fractional-opacity
-// geometry: one walk per crossing, each finding the nearest crossing past
-// the previous one, nearest first, at most ATTENUATION_CROSSINGS of them.
+// geometry: one more walk over [0, t_end) adds each non-target crossing's
+// fixed-point log2 transmittance (the instance record's `log2_transmittance`,
+// rounded on the host) and stops once the sum is below the cutoff. Integer
+// adds make the result independent of the walk's order.
+template <bool W8>
 __device__ inline float shadow_attenuation(const SceneView& s, const SlabRay& r, const WRay& w, float t_end,
                                            uint32_t target_inst, uint32_t target_prim, Stack& st,
                                            uint32_t& nodes_visited) {
-    float attenuation = 1.0f;
-    float pt = -1.0f;
-    uint32_t pi = 0, pp = 0;
-    for (uint32_t round = 0; round < ATTENUATION_CROSSINGS; ++round) {
-        float bound = t_end;
-        bool found = false;
-        uint32_t fi = 0, fp = 0;
-        walk(s.nodes, r, bound, st, nodes_visited, [&](uint32_t ref) {
-            uint32_t first, count;
-            leaf_range(ref, &first, &count);
-            for (uint32_t k = 0; k < count; ++k) {
-                const BvhTriangle tri = s.tris[first + k];
-                if (!(__float_as_uint(tri.v2.w) & TRI_NON_OPAQUE)) continue;
-                const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
-                if (inst == target_inst && prim == target_prim) continue;
-                float t, b1, b2;
-                if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), bound, &t, &b1, &b2)) continue;
-                if (!(t < t_end)) continue;
-                const bool after_prev = t > pt || (t == pt && (inst > pi || (inst == pi && prim > pp)));
-                if (!after_prev) continue;
-                const bool before_best = !found || t < bound || (t == bound && (inst < fi || (inst == fi && prim < fp)));
-                if (!before_best) continue;
-                found = true;
-                bound = t;
-                fi = inst;
-                fp = prim;
-            }
-            return false;
-        });
-        if (!found) break;
-        attenuation *= 1.0f - s.materials[fi].opacity;
-        pt = bound;
-        pi = fi;
-        pp = fp;
-    }
-    return attenuation;
+    int32_t log_sum = 0;
+    float bound = t_end;
+    walk(tree_of<W8>(s), r, bound, st, nodes_visited, [&](uint32_t ref) {
+        uint32_t first, count;
+        leaf_range(ref, &first, &count);
+        for (uint32_t k = 0; k < count; ++k) {
+            const BvhTriangle tri = s.tris[first + k];
+            if (!(__float_as_uint(tri.v2.w) & TRI_NON_OPAQUE)) continue;
+            const uint32_t inst = __float_as_uint(tri.v0.w), prim = __float_as_uint(tri.v1.w);
+            if (inst == target_inst && prim == target_prim) continue;
+            float t, b1, b2;
+            if (!hit_triangle(w, xyz(tri.v0), xyz(tri.v1), xyz(tri.v2), t_end, &t, &b1, &b2)) continue;
+            if (!(t < t_end)) continue;
+            log_sum += int32_t(s.instances[inst].log2_transmittance);
+            if (log_sum < ATTENUATION_CUTOFF) return true;
+        }
+        return false;
+    });
+    return log_sum < ATTENUATION_CUTOFF ? 0.0f : exp2_fixed(log_sum);
 }
 
 }  // namespace cn

@@ -85,6 +85,15 @@ class Scene:
         cam = camera.record()
         check(self._lib.cn_scene_set_camera(self.handle, C.byref(cam)))
 
+    def update(self, data: SceneData, dirty: int):
+        """Scene::update (scene/prep.rs:122-177): rebuild what ``dirty``
+        (``_abi.DIRTY_*`` bits) names from the edited ``data``; the rest stays
+        resident. A rejected edit (CenoteError with CN_E_SCENE) leaves the
+        scene as it was."""
+        desc, keep = data.desc()
+        check(self._lib.cn_scene_update(self.handle, C.byref(desc), dirty))
+        del keep
+
     def bvh(self):
         """The resident BVH: nodes as (N, 32) u32 words (128-byte records)
         and leaf-order triangles as (M, 12) u32 words (48-byte records)."""
@@ -95,6 +104,16 @@ class Scene:
         check(self._lib.cn_scene_bvh(self.handle, nodes.ctypes.data, nn.value, C.byref(nn), tris.ctypes.data,
                                      nt.value, C.byref(nt)))
         return nodes, tris
+
+    def bvh8(self):
+        """The 8-wide traversal nodes as (N, 32) u32 words (128-byte
+        BvhNode8Q records; N = 0 when the scene has no 8-wide tree), over
+        the same leaf-order triangles as ``bvh()``."""
+        nn = C.c_uint32()
+        check(self._lib.cn_scene_bvh8(self.handle, None, 0, C.byref(nn)))
+        nodes = np.zeros((nn.value, 32), np.uint32)
+        check(self._lib.cn_scene_bvh8(self.handle, nodes.ctypes.data, nn.value, C.byref(nn)))
+        return nodes
 
     def info(self) -> _abi.SceneInfo:
         info = _abi.SceneInfo()

@@ -92,6 +92,25 @@ class Material:
     def emitter(emission) -> "Material":
         return replace(Material.matte((0.0, 0.0, 0.0), 0.0), emission=tuple(emission))
 
+    # material.rs:200-245: the same surface with one layer changed.
+    def with_metalness(self, metalness: float) -> "Material":
+        return replace(self, metalness=metalness)
+
+    def with_coat(self, coat_weight: float, coat_roughness: float) -> "Material":
+        return replace(self, coat_weight=coat_weight, coat_roughness=coat_roughness)
+
+    def with_fuzz(self, fuzz_weight: float, fuzz_roughness: float) -> "Material":
+        return replace(self, fuzz_weight=fuzz_weight, fuzz_roughness=fuzz_roughness)
+
+    def with_ior(self, specular_ior: float) -> "Material":
+        return replace(self, specular_ior=specular_ior)
+
+    def with_opacity(self, opacity: float) -> "Material":
+        return replace(self, opacity=opacity)
+
+    def thin(self) -> "Material":
+        return replace(self, thin_walled=True)
+
     def record(self) -> _abi.Material:
         m = _abi.Material()
         for name, _ in _abi.Material._fields_:
@@ -272,6 +291,34 @@ class SceneData:
         d.environment = _abi.Environment(env.shape[1], env.shape[0], env.ctypes.data_as(C.POINTER(C.c_float)))
         d.camera = self.camera.record()
         return d, keep
+
+
+def dirty_bits(old: SceneData, new: SceneData) -> int:
+    """The ``_abi.DIRTY_*`` bits that carry a resident ``old`` scene to ``new``
+    (what changeset.rs:351-356's Dirty set names, at this ABI's grain):
+    geometry when a mesh, an instance's mesh or its transform differs;
+    materials when a material or camera visibility differs; lights, the
+    environment and the camera when they differ."""
+    bits = 0
+    same_meshes = len(old.meshes) == len(new.meshes) and all(
+        np.array_equal(a.positions, b.positions) and np.array_equal(a.normals, b.normals)
+        and np.array_equal(a.indices, b.indices) for a, b in zip(old.meshes, new.meshes))
+    same_placement = len(old.instances) == len(new.instances) and all(
+        a.mesh == b.mesh and np.array_equal(a.transform, b.transform) for a, b in zip(old.instances, new.instances))
+    if not (same_meshes and same_placement):
+        bits |= _abi.DIRTY_GEOMETRY
+    elif any(a.material != b.material or a.camera_visible != b.camera_visible
+             for a, b in zip(old.instances, new.instances)):
+        bits |= _abi.DIRTY_MATERIALS
+    if [(k, tuple(v), tuple(c)) for k, v, c in old.delta_lights] != \
+            [(k, tuple(v), tuple(c)) for k, v, c in new.delta_lights]:
+        bits |= _abi.DIRTY_LIGHTS
+    if old.env_texels is None or new.env_texels is None or old.env_texels.shape != new.env_texels.shape or \
+            not np.array_equal(old.env_texels, new.env_texels):
+        bits |= _abi.DIRTY_ENVIRONMENT
+    if old.camera != new.camera:
+        bits |= _abi.DIRTY_CAMERA
+    return bits
 
 
 def constant_environment(radiance) -> np.ndarray:

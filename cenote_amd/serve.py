@@ -2,13 +2,18 @@
 
 ``Server`` wraps the library's request loop (cn_server_*): requests are
 admitted while their films fit an HBM budget and interleaved into shared
-waves at sample granularity, two waves in flight. ``poisson_requests``
+waves at sample granularity, up to three waves tracing at once on the
+server's own streams. ``poisson_requests``
 draws a seeded open-loop request stream; ``run_served`` replays it against
 a server in real time and ``run_plain`` against the plain form — one
 request at a time, one sample per wave from a reference-sized pool, as the
 reference's Renderer::accumulate loop renders (render/mod.rs:245-259) —
 rendering exactly the samples the served run needed for each request, so
 both forms produce the same images and the comparison is at matched noise.
+``plain_service`` times both plain forms — that reference-shaped loop and
+the best one-request-at-a-time loop (the library's full pool, a request's
+samples stream-ordered) — back to back, and ``fifo_schedule`` places those
+service times under any arrival schedule.
 """
 
 from __future__ import annotations
@@ -227,4 +232,74 @@ def run_plain(ctx: Context, scenes: dict, specs: list, samples: list, capacity: 
             f.close()
         for r in renderers.values():
             r.close()
+    return records
+
+
+def plain_service(ctx: Context, scenes: dict, specs: list, samples: list, capacity: int = 0,
+                  per_sample: bool = False) -> list:
+    """The plain forms, requests back to back in arrival order, one at a
+    time: request k renders ``samples[k]`` samples into a reset film. With
+    ``per_sample`` every sample is its own synchronous wave
+    (Renderer::accumulate, render/mod.rs:245-259) from a ``capacity``-path
+    pool; otherwise the first sample is its own wave (the first frame) and
+    the rest go out stream-ordered from the renderer's pool, synchronized
+    once. Returns each request's service time and the offset of its first
+    frame from the start of its service."""
+    renderers = {}
+    for s in specs:
+        if s.max_bounces not in renderers:
+            renderers[s.max_bounces] = Renderer(ctx, capacity=capacity, max_bounces=s.max_bounces)
+    films = {}
+    out = []
+    try:
+        warm = Film(ctx, 64, 64)
+        for depth, r in renderers.items():  # first launches, outside the timing
+            scene = next(scenes[s.scene] for s in specs if s.max_bounces == depth)
+            warm.reset()
+            r.accumulate(scene, warm)
+        warm.close()
+        if not per_sample:
+            biggest = max(s.width * s.height * n for s, n in zip(specs, samples))
+            for r in renderers.values():
+                r.reserve(min(biggest, r.capacity), aov=True)
+        for s, n in zip(specs, samples):
+            key = (s.width, s.height)
+            if key not in films:
+                films[key] = Film(ctx, s.width, s.height)
+            film = films[key]
+            scene = scenes[s.scene]
+            r = renderers[s.max_bounces]
+            t0 = time.perf_counter()
+            film.reset()
+            scene.set_camera(s.camera)
+            r.accumulate(scene, film)
+            first = time.perf_counter() - t0
+            if n > 1:
+                if per_sample:
+                    for _ in range(n - 1):
+                        r.accumulate(scene, film)
+                else:
+                    r.accumulate(scene, film, count=n - 1)
+            done = time.perf_counter() - t0
+            out.append({"scene": s.scene, "width": s.width, "height": s.height, "samples": n,
+                        "paths": s.width * s.height * n, "first_offset": first, "service": done})
+    finally:
+        for f in films.values():
+            f.close()
+        for r in renderers.values():
+            r.close()
+    return out
+
+
+def fifo_schedule(specs: list, service: list) -> list:
+    """Records of first-in first-out service under the arrival schedule of
+    ``specs``: request k starts once it has arrived and request k - 1 is
+    done, and takes its measured service time."""
+    records = []
+    free = 0.0
+    for s, svc in zip(specs, service):
+        start = max(s.arrival, free)
+        free = start + svc["service"]
+        records.append({"scene": s.scene, "width": s.width, "height": s.height, "samples": svc["samples"],
+                        "arrival": s.arrival, "first_frame": start + svc["first_offset"], "done": free})
     return records

@@ -6,8 +6,10 @@ lock (session.rs:1-51):
 * inputs in — camera, target size, a generation counter, a running flag;
   latest wins, snapshotted once per sample;
 * edits in — change-sets (``apply`` overlays, ``replace`` swaps in a whole
-  scene); they land at the next sample boundary, the scene is rebuilt from
-  the description and accumulation restarts at sample 0 when anything but
+  scene); they land at the next sample boundary, the resident scene is
+  updated in place for exactly what the edit touched (cn_scene_update:
+  materials, lights, the environment, or — for meshes and placements — the
+  geometry and BVH) and accumulation restarts at sample 0 when anything but
   settings changed;
 * frames out — the resolved linear averages (beauty, albedo, normal,
   depth) with their sample count and last sample time, published at most
@@ -25,8 +27,10 @@ import time
 
 import numpy as np
 
+from ._abi import CN_E_SCENE as STATUS_SCENE
+from ._abi import DIRTY_CAMERA, CenoteError
 from .render import Context, Film, LightSampling, Renderer, Scene
-from .scene import CameraData
+from .scene import CameraData, dirty_bits
 from .scenefile import Description, SceneFileError, lower
 
 PUBLISH_INTERVAL = 0.015  # session.rs:65-70
@@ -72,6 +76,7 @@ class Session:
         self._failure = None
         self._renderer = Renderer(ctx, max_bounces=max_bounces, light_sampling=light_sampling)
         self._description = description
+        self._data = data
         self._scene = Scene(ctx, data)
         self._thread = threading.Thread(target=self._run, name="cenote-render", daemon=True)
         self._thread.start()
@@ -149,14 +154,21 @@ class Session:
                 with self._lock_error:
                     self._edit_error = e
                 continue
-            visual = {k: v for k, v in fresh.objects.items() if k != "Settings"} != \
-                     {k: v for k, v in self._description.objects.items() if k != "Settings"}
-            self._description = fresh
-            if visual:
-                scene = Scene(self.ctx, data)
-                self._scene.close()
-                self._scene = scene
+            # Scene::update (scene/prep.rs:122-177): only what the edit
+            # touched is rebuilt; the camera follows the session's inputs.
+            dirty = dirty_bits(self._data, data) & ~DIRTY_CAMERA
+            if dirty:
+                try:
+                    self._scene.update(data, dirty)
+                except CenoteError as e:
+                    if e.status != STATUS_SCENE:
+                        raise
+                    with self._lock_error:
+                        self._edit_error = e
+                    continue
                 changed = True
+            self._description = fresh
+            self._data = data
         return changed
 
     def _loop(self):

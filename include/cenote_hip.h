@@ -215,7 +215,9 @@ typedef struct cn_scene_info {
     uint32_t bvh_max_depth;
     uint32_t bvh_builder;      /* the CN_BVH_* that built it */
     double bvh_build_seconds;
-    double prep_seconds;
+    double prep_seconds;       /* the last create or update, host and device */
+    uint32_t bvh8_node_count;  /* the 8-wide tree over the same leaves (0: none, too deep) */
+    uint32_t bvh8_max_depth;
 } cn_scene_info;
 
 /* Per-bounce queue counts of the most recent wave (the audit of
@@ -268,6 +270,33 @@ int cn_scene_create(cn_context* ctx, const cn_scene_desc* desc, cn_scene** out);
 int cn_scene_destroy(cn_scene* scene);
 /* Camera edits between frames (Scene::camera_mut, scene/mod.rs:391-394). */
 int cn_scene_set_camera(cn_scene* scene, const cn_camera* camera);
+
+/* What an incremental update rebuilds: the parts of the description that
+ * changed since the scene was created or last updated (the Dirty set of
+ * scene/changeset.rs:351-356 folded to this ABI's granularity). */
+enum {
+    CN_DIRTY_CAMERA = 1,       /* the camera */
+    CN_DIRTY_MATERIALS = 2,    /* instance materials and camera visibility (emission included) */
+    CN_DIRTY_LIGHTS = 4,       /* the delta lights */
+    CN_DIRTY_ENVIRONMENT = 8,  /* the environment map */
+    CN_DIRTY_GEOMETRY = 16,    /* meshes, instances or transforms: everything is rebuilt */
+    CN_DIRTY_ALL = 31
+};
+
+/* Scene::update (scene/prep.rs:122-177): rebuilds exactly what `dirty`
+ * names from `desc`, the whole edited description, and leaves the rest of
+ * the residency in place. A material edit rewrites the instance and
+ * material tables, the triangles' flags in place in the BVH and the light
+ * list; a delta-light edit the light list; an environment edit its tables;
+ * each refreshes the light-selection probability. Without
+ * CN_DIRTY_GEOMETRY, `desc` must place the same meshes (by vertex and
+ * triangle count) with the same instances and transforms as the resident
+ * scene, or the call fails with CN_E_SCENE. CN_E_SCENE leaves the scene
+ * untouched (everything fallible runs before the first change); any other
+ * failure is a device fault. The update waits for the device to go idle
+ * before it replaces buffers, so it belongs between waves; the scene then
+ * renders exactly as a scene freshly created from `desc` would. */
+int cn_scene_update(cn_scene* scene, const cn_scene_desc* desc, uint32_t dirty);
 int cn_scene_info_get(cn_scene* scene, cn_scene_info* info);
 /* The BVH as raw records: the full-precision 128-byte nodes (six float4 of
  * child slab bounds, child refs, meta) and the 48-byte leaf-order triangles
@@ -276,6 +305,13 @@ int cn_scene_info_get(cn_scene* scene, cn_scene_info* info);
  * cn_bvh_quantize computes it. */
 int cn_scene_bvh(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t* node_count, void* triangles,
                  uint32_t triangle_capacity, uint32_t* triangle_count);
+/* The 8-wide traversal nodes (128 bytes each: a grid origin and three
+ * power-of-two steps, eight children's byte planes, eight child refs,
+ * depth), collapsed from the same binary tree as the 4-wide nodes over the
+ * same leaf-order triangles. *count = 0 when the scene has no 8-wide tree.
+ * No reference counterpart: the reference's driver owns its acceleration
+ * structure layout (gpu/accel.rs:167-269). */
+int cn_scene_bvh8(cn_scene* scene, void* nodes, uint32_t node_capacity, uint32_t* node_count);
 /* The 64-byte traversal nodes for `count` 128-byte nodes (host memory;
  * no device is involved): per node, an origin, three power-of-two grid
  * steps and each child's planes as bytes rounded outward, so every
@@ -324,6 +360,8 @@ typedef struct cn_launch_shape {
     uint32_t shade_variant;  /* a CN_SHADE_* value */
     uint32_t shade_blocks;   /* most workgroups of shade_surface, 1..65536 */
     uint32_t light_blocks;   /* most 256-lane workgroups of shade_miss, 1..65536 */
+    uint32_t bvh_width;      /* 4 or 8: which of a scene's trees the traversal walks (a scene
+                              * without an 8-wide tree is walked 4-wide) */
 } cn_launch_shape;
 int cn_renderer_get_shape(cn_renderer* renderer, cn_launch_shape* out);
 int cn_renderer_set_shape(cn_renderer* renderer, const cn_launch_shape* shape);
@@ -345,6 +383,16 @@ int cn_renderer_traversal_counts(cn_renderer* renderer, cn_traversal_counts* out
 int cn_renderer_set_profiling(cn_renderer* renderer, int enabled);
 /* Per-kernel device time since the last call; resets the totals. */
 int cn_renderer_kernel_times(cn_renderer* renderer, cn_kernel_times* out);
+/* Measurement only (no reference counterpart): one launch of `blocks`
+ * 256-lane workgroups of a kernel whose waves each issue 8 * iterations VALU
+ * instructions of one kind in eight independent chains — f32 fused
+ * multiply-add, u32 add or f32 max — the yardsticks the SQ instruction
+ * counters of the stage kernels are read against. *ms is the launch's device
+ * time, *valu_per_wave its instruction count per wave (the counters add a
+ * few setup instructions). */
+enum { CN_CALIBRATE_FMA_F32 = 0, CN_CALIBRATE_ADD_U32 = 1, CN_CALIBRATE_MAX_F32 = 2 };
+int cn_calibrate_valu(cn_context* ctx, uint32_t kind, uint32_t iterations, uint32_t blocks, double* ms,
+                      uint64_t* valu_per_wave);
 
 /* Wavefront::trace (wavefront.rs:571-582): one sample of a width x height
  * target into `radiance` (host RGBA f32, width*height*4 floats). */
@@ -413,9 +461,13 @@ int cn_render_batch(cn_renderer* renderer, const cn_view_job* views, uint32_t co
  * requests into shared waves at sample granularity: each step builds one
  * wave in which every request without a sample gets its first one, every
  * other unfinished request one more, and leftover pool room more samples in
- * admission order. Two waves stay in flight. A request's film receives its
- * samples in index order, so its result is bitwise the same alone as in any
- * mix. Times are seconds on the server's steady clock. */
+ * admission order. Successive waves go to three lanes of the server's own
+ * (renderers with their own streams and path pools, copying the given
+ * renderer's depth, light sampling and launch shape), so up to three waves
+ * trace at once; each wave's accumulate waits for the previous wave's. A
+ * request's film receives its samples in index order, so its result is
+ * bitwise the same alone as in any mix. Times are seconds on the server's
+ * steady clock. */
 typedef struct cn_server cn_server;
 
 enum {
@@ -455,8 +507,9 @@ typedef struct cn_server_stats {
     uint64_t hbm_used_bytes;      /* films held by requests */
 } cn_server_stats;
 
-/* wave_paths: most paths per wave (0 = the renderer's capacity, at most
- * 2^24: smaller waves return first frames sooner). */
+/* wave_paths: most paths per wave (0 = 2^24; at most the renderer's
+ * capacity; smaller waves return first frames sooner). Each lane allocates
+ * a pool of that many paths (228 bytes each) at creation. */
 int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t wave_paths,
                      cn_server** out);
 int cn_server_destroy(cn_server* server);

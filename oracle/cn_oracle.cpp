@@ -179,6 +179,55 @@ inline float hash_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint
     return to_unit_float(h);
 }
 
+// Stochastic opacity's per-crossing Bernoulli trial. intersect.slang:64-76
+// draws hashUniform(pixel, sample, bounce, crossing) with `crossing` the
+// candidate's rank in the driver's traversal order; that rank is not
+// reproducible outside the driver, so the build keys the trial on the
+// crossing's identity instead: the same (pixel, sample, bounce) hash chain
+// with (instance, primitive) folded in where the rank went. Every surface
+// then makes one independent trial and the hit is the nearest surface whose
+// trial commits, in any traversal order.
+inline float crossing_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t instance,
+                              uint32_t primitive) {
+    uint32_t h = lowbias32(pixel ^ lowbias32(sample ^ 0x9e3779b9u));
+    h = lowbias32(h ^ (bounce << 8));
+    h = lowbias32(h ^ lowbias32(instance ^ lowbias32(primitive ^ 0x85ebca6bu)));
+    return to_unit_float(h);
+}
+
+// trace_shadow.slang:52-65 multiplies (1 - opacity) per crossing and gives
+// up below 1e-4. The float product's value depends on the order of its
+// factors, so the build keeps it as a sum of base-2 logarithms in 2^-24
+// fixed point (integer adds, the same in any order): per instance
+// log2(1 - opacity) rounded to nearest on that grid, floored at -2^28; the
+// connection is dark once the sum is below ceil(log2(1e-4) * 2^24); a sum
+// maps back to a float by a fixed Taylor polynomial of e^x in double.
+constexpr int32_t LOG2_FIXED_SHIFT = 24;
+constexpr int32_t LOG2_FIXED_FLOOR = -(1 << 28);
+constexpr int32_t ATTENUATION_CUTOFF = -222930820;
+
+int32_t log2_transmittance_fixed(float opacity) {
+    double transmittance = 1.0 - double(opacity);
+    if (!(transmittance > 0.0)) return LOG2_FIXED_FLOOR;
+    if (transmittance >= 1.0) return 0;
+    double units = std::nearbyint(std::log2(transmittance) * double(1 << LOG2_FIXED_SHIFT));
+    return units < double(LOG2_FIXED_FLOOR) ? LOG2_FIXED_FLOOR : int32_t(units);
+}
+
+float exp2_fixed(int32_t L) {
+    int32_t n = L >> LOG2_FIXED_SHIFT;
+    int32_t f = L - n * (1 << LOG2_FIXED_SHIFT);
+    double x = double(f) * (0.6931471805599453 / 16777216.0);
+    // 1/k! for k = 12 down to 0
+    static const double c[13] = {2.08767569878681e-09, 2.505210838544172e-08, 2.755731922398589e-07,
+                                 2.7557319223985893e-06, 2.48015873015873e-05, 0.0001984126984126984,
+                                 0.001388888888888889, 0.008333333333333333, 0.041666666666666664,
+                                 0.16666666666666666, 0.5, 1.0, 1.0};
+    double p = c[0];
+    for (int k = 1; k < 13; ++k) p = p * x + c[k];
+    return std::ldexp(float(p), n < -126 ? -126 : n);
+}
+
 // shaders/rng.slang:140-145
 inline f2 sample_2d(uint32_t pixel, uint32_t sample, uint32_t dimension) {
     uint32_t seed = lowbias32(pixel ^ lowbias32(dimension));
@@ -1076,6 +1125,7 @@ struct Instance {
     uint32_t mesh;
     uint32_t mask;
     bool opaque;
+    int32_t log2_transmittance;  // log2_transmittance_fixed(opacity) for a non-opaque instance
 };
 
 struct MeshData {
@@ -1634,10 +1684,6 @@ inline void add4(f4& a, f3 v, float w) {
 }
 
 constexpr uint32_t ROULETTE_START = 3;  // shade_surface.slang:53
-constexpr uint32_t TRANSPARENT_CAP = 8;  // intersect.slang:23
-// The shadow connection multiplies in the transparency of at most this many
-// fractional-opacity crossings, nearest first.
-constexpr size_t ATTENUATION_CROSSINGS = 16;
 
 // One path of one wave: raygen.slang:64-112, then per bounce
 // intersect.slang:45-88, shade_miss.slang:27-75 or
@@ -1686,13 +1732,8 @@ void trace_path(const orc_scene& s, const Basis& basis, float aperture, uint32_t
         // ---- intersect
         Ray ray{origin, direction, 1e30f};
         uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
-        uint32_t crossings = 0;
-        HitRecord hit = closest_hit(s, ray, mask, [&](uint32_t inst, uint32_t, float) {
-            float opacity = s.materials[inst].opacity;
-            bool commit = crossings >= TRANSPARENT_CAP ||
-                          hash_uniform(pixel, sample, bounce, crossings) < opacity;
-            ++crossings;
-            return commit;
+        HitRecord hit = closest_hit(s, ray, mask, [&](uint32_t inst, uint32_t prim, float) {
+            return crossing_uniform(pixel, sample, bounce, inst, prim) < s.materials[inst].opacity;
         });
 
         if (!hit.hit) {
@@ -1835,11 +1876,9 @@ void trace_path(const orc_scene& s, const Basis& basis, float aperture, uint32_t
 
         // trace_shadow.slang:27-72: visible iff nothing solid is committed
         // within tmax or the committed hit is the target (instance,
-        // primitive). Fractional-opacity crossings that are not the target
-        // attenuate multiplicatively; the reference multiplies them in the
-        // driver's traversal order, which no test pins, so here the product
-        // runs over the crossings in front of the connection's end, in
-        // increasing t — an order any traversal reproduces.
+        // primitive). Every fractional-opacity crossing in front of the
+        // connection's end that is not the target attenuates it; the
+        // product is the order-free fixed-point form above.
         auto traceShadow = [&]() {
             if (!haveShadow) return;
             struct Crossing {
@@ -1855,20 +1894,12 @@ void trace_path(const orc_scene& s, const Basis& basis, float aperture, uint32_t
             });
             if (sh.hit && !(sh.instance == shadowLight && sh.primitive == shadowPrim)) return;
             float tEnd = sh.hit ? sh.t : shadowTmax;
-            std::vector<Crossing> front;
+            int64_t logSum = 0;
             for (const Crossing& c : list) {
-                if (c.t < tEnd) front.push_back(c);
+                if (c.t < tEnd) logSum += s.instances[c.inst].log2_transmittance;
             }
-            std::sort(front.begin(), front.end(), [](const Crossing& a, const Crossing& b) {
-                if (a.t != b.t) return a.t < b.t;
-                if (a.inst != b.inst) return a.inst < b.inst;
-                return a.prim < b.prim;
-            });
-            float attenuation = 1.0f;
-            for (size_t k = 0; k < front.size() && k < ATTENUATION_CROSSINGS; ++k) {
-                attenuation *= 1.0f - s.materials[front[k].inst].opacity;
-            }
-            if (attenuation < 1e-4f) return;
+            if (logSum < ATTENUATION_CUTOFF) return;
+            float attenuation = exp2_fixed(int32_t(logSum));
             add4(out.radiance, shadowContribution * attenuation, 0.0f);
         };
 
@@ -1991,6 +2022,12 @@ float orc_hash_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_
     return hash_uniform(pixel, sample, bounce, crossing);
 }
 
+float orc_crossing_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t instance, uint32_t primitive) {
+    return crossing_uniform(pixel, sample, bounce, instance, primitive);
+}
+int32_t orc_log2_transmittance_fixed(float opacity) { return log2_transmittance_fixed(opacity); }
+float orc_exp2_fixed(int32_t sum) { return exp2_fixed(sum); }
+
 uint32_t orc_lowbias32(uint32_t x) { return lowbias32(x); }
 uint32_t orc_laine_karras(uint32_t x, uint32_t seed) { return laine_karras(x, seed); }
 uint32_t orc_sobol(uint32_t index, uint32_t axis) { return sobol(index, axis); }
@@ -2072,6 +2109,7 @@ orc_scene* orc_scene_create(const cn_scene_desc* desc, const float* bsdf_tables)
         inst.mesh = src.mesh;
         inst.mask = src.camera_visible ? MASK_ALL : (MASK_ALL & ~MASK_CAMERA);
         inst.opaque = src.material.opacity >= 1.0f;
+        inst.log2_transmittance = inst.opaque ? 0 : log2_transmittance_fixed(src.material.opacity);
         inst.light = LIGHT_NONE;
         Material m = lift_material(src.material);
         s->materials.push_back(m);
@@ -2175,6 +2213,33 @@ void orc_accumulate_rows(orc_scene* scene, uint32_t width, uint32_t height, uint
     for (auto& th : pool) th.join();
 }
 
+void orc_accumulate_pixels(orc_scene* scene, uint32_t width, uint32_t height, const uint32_t* pixels,
+                           uint32_t pixel_count, uint32_t first, uint32_t count, uint32_t max_bounces,
+                           uint32_t light_sampling, float* beauty_sum, int threads) {
+    // accumulate.slang:46-67 for an arbitrary list of pixel indices of a
+    // width x height wave (scattered tiles): sums in sample order, one
+    // 4-float entry per listed pixel.
+    float aperture;
+    Basis basis = camera_basis(scene->camera, float(width) / float(height), &aperture);
+    if (threads < 1) threads = 1;
+    auto worker = [&](int id) {
+        for (size_t k = size_t(id); k < pixel_count; k += size_t(threads)) {
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t sample = first + i;
+                PixelOut o;
+                trace_path(*scene, basis, aperture, width, height, pixels[k], sample, max_bounces, light_sampling,
+                           false, o);
+                float c[4] = {o.radiance.x, o.radiance.y, o.radiance.z, o.radiance.w};
+                accumulate4(c, beauty_sum + 4 * k, 0, sample == 0);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+}
+
 void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t sample,
                     uint32_t* instance, uint32_t* primitive, float* bary, int threads) {
     float aperture;
@@ -2211,15 +2276,10 @@ void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t 
                     origin = origin + offset;
                     direction = normalize(target - offset);
                 }
-                uint32_t crossings = 0;
                 HitRecord h = closest_hit(*scene, Ray{origin, direction, 1e30f}, MASK_CAMERA,
-                                          [&](uint32_t inst, uint32_t, float) {
-                                              float opacity = scene->materials[inst].opacity;
-                                              bool commit =
-                                                  crossings >= TRANSPARENT_CAP ||
-                                                  hash_uniform(pixel, sample, 0, crossings) < opacity;
-                                              ++crossings;
-                                              return commit;
+                                          [&](uint32_t inst, uint32_t prim, float) {
+                                              return crossing_uniform(pixel, sample, 0, inst, prim) <
+                                                     scene->materials[inst].opacity;
                                           });
                 instance[pixel] = h.hit ? h.instance : 0xffffffffu;
                 primitive[pixel] = h.hit ? h.primitive : 0xffffffffu;

@@ -28,6 +28,12 @@ void orc_sample_2d(uint32_t pixel, uint32_t sample, uint32_t dimension, float ou
 float orc_sample_1d(uint32_t pixel, uint32_t sample, uint32_t dimension);
 /* shaders/rng.slang:128-132 */
 float orc_hash_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t crossing);
+/* Stochastic opacity's per-crossing trial and the shadow attenuation's
+ * fixed-point log form (the order-free restatements of intersect.slang:64-76
+ * and trace_shadow.slang:52-65). */
+float orc_crossing_uniform(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t instance, uint32_t primitive);
+int32_t orc_log2_transmittance_fixed(float opacity);
+float orc_exp2_fixed(int32_t sum);
 /* The raw u32 pieces, for known-answer checks. */
 uint32_t orc_lowbias32(uint32_t x);
 uint32_t orc_laine_karras(uint32_t x, uint32_t seed);
@@ -91,6 +97,12 @@ void orc_accumulate(orc_scene* scene, uint32_t width, uint32_t height, uint32_t 
 void orc_accumulate_rows(orc_scene* scene, uint32_t width, uint32_t height, uint32_t row0, uint32_t row1,
                          uint32_t first, uint32_t count, uint32_t max_bounces, uint32_t light_sampling,
                          float* beauty_sum, int threads);
+
+/* The beauty sums of samples first .. first+count-1 at each of `pixel_count`
+ * listed pixel indices of a width x height wave, one 4-float sum per entry. */
+void orc_accumulate_pixels(orc_scene* scene, uint32_t width, uint32_t height, const uint32_t* pixels,
+                           uint32_t pixel_count, uint32_t first, uint32_t count, uint32_t max_bounces,
+                           uint32_t light_sampling, float* beauty_sum, int threads);
 
 /* The bounce-0 closest hit per pixel (shaders/intersect.slang:45-88). */
 void orc_first_hits(orc_scene* scene, uint32_t width, uint32_t height, uint32_t sample,

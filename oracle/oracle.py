@@ -52,6 +52,12 @@ def lib() -> C.CDLL:
     L.orc_sample_1d.restype = C.c_float
     L.orc_hash_uniform.argtypes = [u, u, u, u]
     L.orc_hash_uniform.restype = C.c_float
+    L.orc_crossing_uniform.argtypes = [u, u, u, u, u]
+    L.orc_crossing_uniform.restype = C.c_float
+    L.orc_log2_transmittance_fixed.argtypes = [C.c_float]
+    L.orc_log2_transmittance_fixed.restype = C.c_int32
+    L.orc_exp2_fixed.argtypes = [C.c_int32]
+    L.orc_exp2_fixed.restype = C.c_float
     for name in ("orc_lowbias32", "orc_reverse_bits"):
         getattr(L, name).argtypes = [u]
         getattr(L, name).restype = u
@@ -78,6 +84,7 @@ def lib() -> C.CDLL:
     L.orc_trace_rows.argtypes = [vp, u, u, u, u, u, u, u, C.c_int, fp, fp, fp, fp, C.c_int]
     L.orc_accumulate.argtypes = [vp, u, u, u, u, u, u, fp, fp, fp, fp, C.c_int]
     L.orc_accumulate_rows.argtypes = [vp, u, u, u, u, u, u, u, u, fp, C.c_int]
+    L.orc_accumulate_pixels.argtypes = [vp, u, u, up, u, u, u, u, u, fp, C.c_int]
     L.orc_first_hits.argtypes = [vp, u, u, u, up, up, fp, C.c_int]
     L.orc_env_sample.argtypes = [vp, fp, u, fp]
     _lib = L
@@ -213,6 +220,17 @@ class OracleScene:
         sums = np.zeros((row1 - row0, width, 4), np.float32)
         lib().orc_accumulate_rows(self.handle, width, height, row0, row1, first, count, max_bounces,
                                   light_sampling, _fp(sums), threads)
+        return sums
+
+    def accumulate_pixels(self, width, height, pixels, first, count, max_bounces=8, light_sampling=0,
+                          threads=None):
+        """Beauty sums of samples first..first+count-1 at the listed pixel
+        indices (n,), returned as (n, 4)."""
+        threads = threads or threads_default()
+        pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+        sums = np.zeros((len(pixels), 4), np.float32)
+        lib().orc_accumulate_pixels(self.handle, width, height, pixels.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    len(pixels), first, count, max_bounces, light_sampling, _fp(sums), threads)
         return sums
 
     def sample_environment(self, u: np.ndarray):

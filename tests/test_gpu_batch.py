@@ -145,3 +145,44 @@ def test_batch_mixes_views_of_different_depths(ctx, scenes):
     r.close()
     cs.set_camera(cornell.camera)
     hs.set_camera(chart.camera)
+
+
+def test_the_64_view_batch_matches_one_view_at_a_time(ctx):
+    """The batched workload at its stated size: 64 seeded views of the chart
+    and the 2M-triangle scenes (128^2..1024^2, aspect 1..16/9, their own
+    depth caps), with each view's 1..64 spp capped at 4 to bound the run.
+    One render_batch with an autotuned launch shape gives every film the
+    bitwise sums of that view rendered alone (all of its samples
+    stream-ordered, the renderer's own pool)."""
+    from cenote_amd import autotune
+
+    data = {n: workloads.build(n) for n in ("chart", "heavy")}
+    scenes = {n: cn.Scene(ctx, d) for n, d in data.items()}
+    views = workloads.batched_views({n: d.camera for n, d in data.items()})
+    assert len(views) == 64 and {v[0] for v in views} == {"chart", "heavy"}
+    views = [(n, w, h, min(spp, 4), depth, cam) for n, w, h, spp, depth, cam in views]
+    print(f"64 views: {sum(w * h * s for _, w, h, s, _, _ in views) / 1e6:.1f} M camera paths", flush=True)
+    alone = {d: cn.Renderer(ctx, max_bounces=d) for d in (12, 16)}
+    want = []
+    for name, w, h, spp, depth, cam in views:
+        scenes[name].set_camera(cam)
+        film = cn.Film(ctx, w, h)
+        alone[depth].accumulate(scenes[name], film, count=spp)
+        want.append(film.sums())
+        film.close()
+    for r in alone.values():
+        r.close()
+    r = cn.Renderer(ctx, max_bounces=16)
+    tuned = autotune.tune(r, scenes["heavy"], 512, 288, samples=2, repeats=1)
+    print(f"autotuned shape {tuned['shape']}", flush=True)
+    films = [cn.Film(ctx, w, h) for _, w, h, _, _, _ in views]
+    r.render_batch([(scenes[n], f, cam, spp, depth) for (n, _, _, spp, depth, cam), f in zip(views, films)])
+    for k, (f, (name, w, h, spp, _, _)) in enumerate(zip(films, views)):
+        assert f.samples == spp
+        got = f.sums()
+        for g, x in zip(got, want[k]):
+            assert np.array_equal(g, x, equal_nan=True), f"view {k} ({name} {w}x{h}, {spp} spp) differs"
+        f.close()
+    r.close()
+    for s in scenes.values():
+        s.close()

@@ -50,24 +50,29 @@ CASES = {
 }
 
 
-def built(ctx, data, builder):
+def built(ctx, data, builder, wide=False):
     scene = cn.Scene(ctx, data, bvh_builder=builder)
     nodes, tris = scene.bvh()
     info = scene.info()
+    nodes8 = scene.bvh8()
     scene.close()
-    return nodes, tris, info
+    return (nodes, tris, info, nodes8) if wide else (nodes, tris, info)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_device_and_host_builders_grow_the_same_bvh(ctx, name):
     data = CASES[name]()
-    dn, dt, di = built(ctx, data, _abi.BVH_DEVICE)
-    hn, ht, hi = built(ctx, data, _abi.BVH_HOST)
+    print(f"builders: {name}", flush=True)
+    dn, dt, di, d8 = built(ctx, data, _abi.BVH_DEVICE, wide=True)
+    hn, ht, hi, h8 = built(ctx, data, _abi.BVH_HOST, wide=True)
     assert di.bvh_builder == _abi.BVH_DEVICE and hi.bvh_builder == _abi.BVH_HOST
     assert dn.shape == hn.shape and dt.shape == ht.shape
     assert int(np.count_nonzero(dn != hn)) == 0
     assert int(np.count_nonzero(dt != ht)) == 0
     assert di.bvh_max_depth == hi.bvh_max_depth
+    # the 8-wide nodes too, collapsed and quantized on the device vs the host
+    assert d8.shape == h8.shape and int(np.count_nonzero(d8 != h8)) == 0
+    assert di.bvh8_max_depth == hi.bvh8_max_depth
     # Every triangle appears exactly once in leaf order.
     inst_prim = dt[:, 3].astype(np.uint64) << np.uint64(32) | dt[:, 7].astype(np.uint64)
     assert np.unique(inst_prim).size == data.triangle_count
@@ -75,11 +80,35 @@ def test_device_and_host_builders_grow_the_same_bvh(ctx, name):
 
 def test_heavy_scene_builds_the_same_on_the_device(ctx):
     data = workloads.heavy()
-    dn, dt, di = built(ctx, data, _abi.BVH_DEVICE)
-    hn, ht, hi = built(ctx, data, _abi.BVH_HOST)
+    dn, dt, di, d8 = built(ctx, data, _abi.BVH_DEVICE, wide=True)
+    hn, ht, hi, h8 = built(ctx, data, _abi.BVH_HOST, wide=True)
     assert int(np.count_nonzero(dn != hn)) == 0
     assert int(np.count_nonzero(dt != ht)) == 0
+    assert int(np.count_nonzero(d8 != h8)) == 0
     print(f"heavy BVH build: device {di.bvh_build_seconds:.3f} s, host {hi.bvh_build_seconds:.3f} s")
+
+
+@pytest.mark.parametrize("name", ["cornell", "chart"])
+def test_device_and_host_built_scenes_render_the_same_films(ctx, name):
+    """The device's own quantized 4-wide nodes (never read back) walk to the
+    same films as the host-quantized ones, at both widths."""
+    data = workloads.build(name)
+    sums = {}
+    for builder in (_abi.BVH_DEVICE, _abi.BVH_HOST):
+        scene = cn.Scene(ctx, data, bvh_builder=builder)
+        for width in (4, 8):
+            r = cn.Renderer(ctx, max_bounces=8)
+            r.shape = {"bvh_width": width}
+            film = cn.Film(ctx, 96, 64)
+            r.accumulate(scene, film, count=2)
+            sums[(builder, width)] = film.sums()
+            film.close()
+            r.close()
+        scene.close()
+    want = sums[(_abi.BVH_HOST, 4)]
+    for key, got in sums.items():
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w, equal_nan=True), key
 
 
 @pytest.mark.parametrize("name", ["chart", "soup_wide_100k", "degenerate_signed_zeros"])

@@ -184,3 +184,54 @@ def test_invalid_requests_are_rejected(ctx, scenes):
         sv.submit(handles["cornell"], cam, 64, 64, 4, rel_mse=-1.0)
     sv.close()
     r.close()
+
+
+def test_the_serving_mix_at_full_resolution(ctx):
+    """The serving workload's request mix at its stated sizes: the resident
+    Cornell, chart and 2M-triangle scenes at 256^2, 1280x720 and 1920x1080,
+    each scene at its own depth cap, arriving together under a budget that
+    holds only a few films, in 2^22-path waves. Every result is bitwise the
+    same alone as in the mix, and a noise-target request in the mix equals
+    a plain render of the samples it stopped at."""
+    data = {n: workloads.build(n) for n in ("cornell", "chart", "heavy")}
+    handles = {n: cn.Scene(ctx, d) for n, d in data.items()}
+    depth = {n: workloads.CONFIGS[n]["max_bounces"] for n in data}
+    mix = [("heavy", 1920, 1080, 3), ("chart", 1280, 720, 6), ("cornell", 256, 256, 17), ("heavy", 1920, 1080, 1),
+           ("cornell", 256, 256, 2), ("chart", 1280, 720, 2), ("heavy", 1280, 720, 4), ("chart", 1920, 1080, 2)]
+    film_1080p = 1920 * 1080 * 56
+    budget = 3 * film_1080p
+
+    def run(requests, noisy=None):
+        r = cn.Renderer(ctx, max_bounces=16)
+        sv = serve.Server(r, budget, 1 << 22)
+        ids = [sv.submit(handles[n], data[n].camera, w, h, spp, max_bounces=depth[n]) for n, w, h, spp in requests]
+        nid = None
+        if noisy is not None:
+            n, w, h = noisy
+            nid = sv.submit(handles[n], data[n].camera, w, h, 256, rel_mse=1e-2, max_bounces=depth[n])
+        out, noise = [None] * len(ids), None
+        busy = 1
+        while busy:
+            busy = sv.step()
+            assert sv.stats().hbm_used_bytes <= budget
+            for k, (rid, (_, w, h, spp)) in enumerate(zip(ids, requests)):
+                if out[k] is None and sv.status(rid).state == _abi.REQUEST_DONE:
+                    assert sv.status(rid).samples == spp
+                    out[k] = sv.result(rid, w, h)
+            if nid is not None and noise is None and sv.status(nid).state == _abi.REQUEST_DONE:
+                noise = (sv.status(nid).samples, sv.result(nid, noisy[1], noisy[2]))
+        sv.close()
+        r.close()
+        return out, noise
+
+    print("serving mix: all requests together", flush=True)
+    mixed, (noise_samples, noise_result) = run(mix, noisy=("cornell", 256, 256))
+    for k, req in enumerate(mix):
+        print(f"serving mix: {req} alone", flush=True)
+        alone, _ = run([req])
+        assert np.array_equal(alone[0], mixed[k], equal_nan=True), req
+    assert 2 <= noise_samples <= 256
+    want = plain_average(ctx, handles["cornell"], data["cornell"].camera, 256, 256, noise_samples, depth["cornell"])
+    assert np.array_equal(want, noise_result, equal_nan=True)
+    for h in handles.values():
+        h.close()

@@ -10,7 +10,8 @@ from cenote_amd import workloads
 from cenote_amd.scene import Material, OpenPBR, acescg_from_rec709
 from oracle import oracle as orc
 
-from reference_scenes import agreement_scene, aov_quad_scene, distant_light_scene, point_light_scene
+from reference_scenes import (FURNACE_BOUND, FURNACE_CONFIGS, agreement_scene, aov_quad_scene, color_bleed_scene,
+                              distant_light_scene, glass_sphere_scene, point_light_scene, redness)
 
 BSDF_ONLY, NEE_ONLY, MIS = 1, 2, 0
 
@@ -61,30 +62,70 @@ def test_a_point_light_is_analytically_exact():
     assert np.allclose(img[..., :3], 1.0, atol=5e-3)
 
 
-WHITE = (1.0, 1.0, 1.0)
-FURNACE_MATRIX = {
-    "metal": Material.metal(WHITE, 0.5),
-    "smooth_metal": Material.metal(WHITE, 0.05),
-    "glossy": Material.glossy(WHITE, 0.0, 0.3),
-    "ior_2": OpenPBR(base_color=WHITE, specular_ior=2.0, specular_roughness=0.5).lower(),
-    "coat": OpenPBR(base_color=WHITE, coat_weight=1.0, coat_roughness=0.2).lower(),
-    "fuzz": OpenPBR(base_color=WHITE, fuzz_weight=1.0).lower(),
-    "full_stack": OpenPBR(base_color=WHITE, base_metalness=0.5, coat_weight=0.7, fuzz_weight=0.5,
-                          specular_roughness=0.4).lower(),
-    "glass": Material.glass(0.3, 1.5),
-    "thin_glass": OpenPBR(transmission_weight=1.0, geometry_thin_walled=True, specular_roughness=0.2).lower(),
-    "half_opacity": OpenPBR(base_color=WHITE, specular_weight=0.0, geometry_opacity=0.5).lower(),
-}
-
-
-@pytest.mark.parametrize("name", sorted(FURNACE_MATRIX))
+@pytest.mark.parametrize("name", list(FURNACE_CONFIGS))
 def test_openpbr_furnace_matrix(name):
     """render/tests.rs:422-488: a white material of any construction returns
-    the sky, within the reference's 1.5%."""
-    scene = orc.OracleScene(workloads.furnace(FURNACE_MATRIX[name]))
-    b, _, _, _ = scene.accumulate(24, 24, 0, 64, max_bounces=64)
-    mean = b[..., :3].mean() / 64
-    assert abs(mean - 0.5) / 0.5 < 0.015, (name, mean)
+    the sky: within 1e-3 wherever the tabulated energy terms allow, within
+    the measured table residual elsewhere (FURNACE_BOUND)."""
+    scene = orc.OracleScene(workloads.furnace(FURNACE_CONFIGS[name]))
+    b, _, _, _ = scene.accumulate(32, 32, 0, 1024)
+    rel = (b[..., :3].mean() / 1024 - 0.5) / 0.5
+    assert abs(rel) < FURNACE_BOUND[name], (name, rel)
+
+
+@pytest.mark.parametrize("depth,bound", [(16, 0.015), (255, 1e-3)])
+def test_the_glass_furnace_closes(depth, bound):
+    """render/tests.rs:499-540: a closed rough-glass sphere. At the
+    reference's depth cap of 16 the paths cut short inside the glass leave
+    -1.1% (the reference's bound is 1.5%); at 255 bounces the residual is
+    -5e-4, inside 1e-3."""
+    scene = orc.OracleScene(glass_sphere_scene(Material.glass(0.2, 1.5)))
+    b, _, _, _ = scene.accumulate(32, 32, 0, 512, max_bounces=depth)
+    rel = (b[..., 0].mean() / 512 - 0.5) / 0.5
+    assert abs(rel) < bound, rel
+
+
+def test_interior_absorption_is_spectral():
+    """render/tests.rs:541-592: red absorbed along interior chords, green
+    (absorption-free) still closes."""
+    from dataclasses import replace
+
+    m = replace(Material.glass(0.2, 1.5), transmission_color=(0.4, 1.0, 1.0), transmission_depth=1.0)
+    b, _, _, _ = orc.OracleScene(glass_sphere_scene(m)).accumulate(32, 32, 0, 256, max_bounces=255)
+    red, green = b[..., 0].mean() / 256, b[..., 1].mean() / 256
+    assert abs(green - 0.5) / 0.5 < 2e-3, green
+    assert 0.2 * 0.5 < red < 0.9 * 0.5, red
+
+
+def test_coat_darkening_darkens_a_gray_base():
+    """render/tests.rs:627-647: darkening 0 -> 1 under a 1.6-IOR coat over a
+    0.5 Lambert base lands near the spec's 0.52."""
+    from dataclasses import replace
+
+    means = []
+    for darkening in (0.0, 1.0):
+        m = replace(Material.matte((0.5, 0.5, 0.5), 0.0).with_coat(1.0, 0.1), coat_darkening=darkening)
+        b, _, _, _ = orc.OracleScene(workloads.furnace(m)).accumulate(16, 16, 0, 32)
+        means.append(b[..., 0].mean() / 32)
+    assert 0.45 < means[1] / means[0] < 0.75, means
+
+
+def test_specular_guides_pass_through_to_what_mirrors_show():
+    """render/tests.rs:1233-1279: a mirror records 0.035/0.15 of its normal
+    (the roughness floor into the guide ramp), roughness 0.5 all of it."""
+    for roughness, expected in ((0.0, 0.035 / 0.15), (0.5, 1.0)):
+        scene = orc.OracleScene(workloads.furnace(Material.metal((1.0, 1.0, 1.0), roughness)))
+        _, a, n, d = scene.accumulate(16, 16, 0, 2)
+        assert abs(a[8, 8, 0] / 2 - 1.0) < 1e-3
+        assert np.isfinite(d[8, 8]) and d[8, 8] > 0.0
+        assert abs(n[8, 8, 1] / 2 - expected) < 1e-3, (roughness, n[8, 8, 1] / 2)
+
+
+def test_indirect_light_bleeds_color():
+    """render/tests.rs:1051-1116: floor beside the terracotta sphere is
+    redder than the far corner."""
+    b, _, _, _ = orc.OracleScene(color_bleed_scene()).accumulate(64, 64, 0, 32)
+    assert redness(b, 30, 53) > 1.05 * redness(b, 2, 60)
 
 
 def test_aovs_record_albedo_normal_and_depth():
