@@ -428,8 +428,7 @@ def serving_bench(ctx, rates=(10.0, 50.0, 100.0, 200.0), requests=48, data=None)
     data, scenes = _scenes(ctx, ("cornell", "chart", "heavy"), data)
     depths = {n: workloads.CONFIGS[n]["max_bounces"] for n in scenes}
     cams = {n: data[n].camera for n in scenes}
-    r = cn.Renderer(ctx, max_bounces=16)
-    r.reserve(1 << 24, aov=True)  # the server's waves hold at most 2^24 paths
+    r = cn.Renderer(ctx, max_bounces=16)  # depth and launch shape for the server's lanes
     budget = 8 << 30
     sv = serve.Server(r, budget)
     for n in scenes:  # warm: one small request through every scene

@@ -129,7 +129,7 @@ int ensure_slots(cn_renderer* r, size_t slots, bool aov) {
 }
 
 int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena,
-                 hipEvent_t accumulate_after) {
+                 const hipEvent_t* after, uint32_t after_count) {
     if (jobs.empty()) return CN_OK;
     if (jobs.size() > MAX_JOBS) return fail(CN_E_INVALID, "a wave holds at most %u jobs", MAX_JOBS);
     for (HostJob& j : jobs) {
@@ -203,7 +203,7 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
     // table (stream order keeps a wave's table until its kernels are done),
     // so a captured launch sequence replays for any wave of the same shape.
     // (A wave that waits on another stream's event is launched directly.)
-    const bool graph = r->graphs && !r->profiler && !r->counting && !accumulate_after;
+    const bool graph = r->graphs && !r->profiler && !r->counting && after_count == 0;
     JobRecord* dev = r->jobs.ptr + (graph ? 0 : size_t(e) * MAX_JOBS);
     uint32_t slot_base = 0, pixel_base = 0;
     for (size_t i = 0; i < jobs.size(); ++i) {
@@ -272,7 +272,7 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
             }
         }
         if (films) {
-            if (accumulate_after) CN_HIP(hipStreamWaitEvent(r->stream, accumulate_after, 0));
+            for (uint32_t k = 0; k < after_count; ++k) CN_HIP(hipStreamWaitEvent(r->stream, after[k], 0));
             CN_HIP(launch_accumulate(r->stream, dev, uint32_t(jobs.size()), tg, uint32_t(pixels), r->profiler.get()));
         }
         return CN_OK;

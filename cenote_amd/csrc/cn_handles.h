@@ -179,12 +179,12 @@ struct WaveArena {
 // Enqueue one wave of jobs on the renderer's stream (no host sync unless a
 // buffer must grow or a job-table ring entry is still in flight). With an
 // arena the wave runs in it and nothing is allocated; a wave of more slots
-// than the arena holds fails with CN_E_INVALID. With `accumulate_after`,
-// the wave's accumulate waits for that event (a wave on another stream
-// that folds into the same films), so films still see their samples in
+// than the arena holds fails with CN_E_INVALID. The wave's accumulate
+// waits for the `after_count` events of `after` (waves on other streams
+// that fold into the same films), so films still see their samples in
 // order while the waves' tracing overlaps.
 int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const WaveArena* arena = nullptr,
-                 hipEvent_t accumulate_after = nullptr);
+                 const hipEvent_t* after = nullptr, uint32_t after_count = 0);
 
 // A film job for `samples` more samples of `camera` on `s`, continuing at
 // the film's sample count.

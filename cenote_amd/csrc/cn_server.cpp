@@ -4,17 +4,20 @@
 // progressive Session (render/session.rs:480-489), which renders one view;
 // this loop generalises it to many concurrent requests.
 //
-// Per step: admit queued requests (FIFO) while their films fit the budget;
-// build one wave — every admitted request that has no sample yet gets one
-// first (the first frame), then every unfinished request one more, then
-// leftover pool room goes to extra samples in admission order; enqueue it
-// on the next of LANES lanes — renderers of the server's own, each with its
-// stream and path pool — so that up to LANES waves trace at once and one
-// wave's sparse deep bounces overlap the next waves' full ones; retire the
-// oldest wave once LANES are in flight. Every wave's accumulate waits for
-// the previous wave's (an event across lanes), so a request's film sees its
-// samples in index order and its result is bitwise the same whether it is
-// served alone or in any mix.
+// The server runs its waves on LANES lanes — renderers of its own, each with
+// its stream and path pool, one wave in flight each. Lane 0 carries first
+// frames: a wave of one sample for every admitted request that has none
+// yet, at most wave_paths paths, launched as soon as the lane is free so
+// that a new request's first frame never queues behind bulk work. The other
+// lanes carry bulk waves of up to BULK_FACTOR x wave_paths paths: one more
+// sample for every request whose first sample is out, then leftover room as
+// extra samples in admission order. Bulk waves are large, so their sparse
+// deep bounces are a small share of them, and the lanes overlap one wave's
+// deep bounces with the others' full ones. A wave's accumulate waits for the
+// earlier waves that fold into any of its films (events across lanes), so
+// every film sees its samples in index order and a result is bitwise the
+// same whether the request is served alone or in any mix. Waves retire as
+// they complete, in any order.
 
 #include <hip/hip_runtime.h>
 
@@ -24,6 +27,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "cn_handles.h"
@@ -33,7 +37,8 @@ using namespace cn;
 namespace {
 
 constexpr uint32_t NOISE_BLOCKS = 64;  // partial sums per noise estimate
-constexpr uint32_t LANES = 3;          // waves tracing at once, one per lane
+constexpr uint32_t LANES = 3;          // lane 0: first frames; lanes 1 and 2: bulk waves
+constexpr uint32_t BULK_FACTOR = 4;    // a bulk wave holds up to this many times wave_paths
 
 size_t film_bytes(uint32_t width, uint32_t height, bool noise) {
     // beauty, albedo, normal sums (16 B each) + depth (4 B) [+ moments (4 B)]
@@ -51,6 +56,7 @@ struct Request {
     bool converged = false;  // the noise target was met
     double noise = -1.0;     // last relMSE estimate
     size_t bytes = 0;
+    uint64_t last_wave = 0;  // serial of the last wave enqueued for this film (0: none)
 };
 
 struct WavePart {
@@ -60,6 +66,8 @@ struct WavePart {
 };
 
 struct InFlight {
+    uint64_t serial = 0;
+    uint32_t lane = 0;
     hipEvent_t event = nullptr;
     uint32_t ring = 0;  // noise-partials ring entry
     std::vector<WavePart> parts;
@@ -71,9 +79,11 @@ struct InFlight {
 struct cn_server {
     cn_renderer* r = nullptr;           // the caller's renderer: depth, light sampling, launch shape
     std::vector<cn_renderer*> lanes;    // the server's own renderers, one wave in flight each
-    uint32_t lane_next = 0;
+    bool lane_busy[LANES] = {};
+    uint64_t wave_serial = 0;
     uint64_t budget = 0;
-    uint32_t wave_paths = 0;
+    uint32_t wave_paths = 0;  // first-frame waves
+    uint32_t bulk_paths = 0;  // bulk waves
     std::chrono::steady_clock::time_point t0;
     std::map<uint64_t, Request> requests;
     std::deque<uint64_t> queue;    // waiting for admission, FIFO
@@ -84,12 +94,11 @@ struct cn_server {
     std::multimap<std::pair<uint32_t, uint32_t>, cn_film*> spare;
     uint64_t spare_bytes = 0;
     uint64_t next_id = 1;
-    std::deque<InFlight> inflight;
+    std::vector<InFlight> inflight;  // in enqueue order
     std::vector<hipEvent_t> free_events;
-    // Noise partials: a ring of LANES + 1 entries of MAX_JOBS estimates.
+    // Noise partials: per lane, MAX_JOBS estimates of NOISE_BLOCKS partial sums.
     DevBuf<double> partials;
     double* host_partials = nullptr;  // pinned
-    uint32_t ring_next = 0;
     uint64_t progress = 0;  // admissions + waves enqueued + waves retired
     cn_server_stats stats{};
 
@@ -100,7 +109,9 @@ struct cn_server {
 
 namespace {
 
-constexpr uint32_t NOISE_RING = LANES + 1;
+// Noise partials: one entry of MAX_JOBS estimates per lane (a lane holds
+// one wave in flight).
+constexpr uint32_t NOISE_RING = LANES;
 
 int admit(cn_server* sv) {
     while (!sv->queue.empty() && sv->active.size() < MAX_JOBS) {
@@ -147,11 +158,13 @@ int admit(cn_server* sv) {
 
 bool wants_more(const Request& q) { return !q.converged && q.scheduled < q.desc.spp; }
 
-int retire_one(cn_server* sv) {
-    InFlight w = std::move(sv->inflight.front());
-    sv->inflight.pop_front();
+// Retires in-flight wave k, whose event has completed.
+int retire(cn_server* sv, size_t k) {
+    InFlight w = std::move(sv->inflight[k]);
+    sv->inflight.erase(sv->inflight.begin() + ptrdiff_t(k));
     CN_HIP(hipEventSynchronize(w.event));
     sv->free_events.push_back(w.event);
+    sv->lane_busy[w.lane] = false;
     sv->progress += 1;
     const double t = sv->now();
     sv->stats.waves += 1;
@@ -159,7 +172,9 @@ int retire_one(cn_server* sv) {
     const double* noise = sv->host_partials + size_t(w.ring) * MAX_JOBS * NOISE_BLOCKS;
     for (const WavePart& part : w.parts) {
         Request& q = sv->requests[part.id];
-        q.completed = part.samples_after;
+        // A film's waves complete in enqueue order (each accumulate waits
+        // for the film's previous wave), so the count only grows.
+        q.completed = std::max(q.completed, part.samples_after);
         if (q.first_frame < 0.0 && q.completed >= 1) q.first_frame = t;
         if (part.noise_slot >= 0) {
             double sum = 0.0;
@@ -184,12 +199,34 @@ int retire_one(cn_server* sv) {
     return CN_OK;
 }
 
-int enqueue(cn_server* sv) {
-    cn_renderer* r = sv->lanes[sv->lane_next];
+// Retires every in-flight wave that has completed; *count gets how many.
+int retire_ready(cn_server* sv, uint32_t* count) {
+    *count = 0;
+    for (size_t k = 0; k < sv->inflight.size();) {
+        const hipError_t q = hipEventQuery(sv->inflight[k].event);
+        if (q == hipErrorNotReady) {
+            ++k;
+            continue;
+        }
+        CN_HIP(q);
+        if (int e = retire(sv, k)) return e;
+        *count += 1;
+    }
+    return CN_OK;
+}
+
+// Builds and enqueues one wave on `lane`: with `first`, the first sample of
+// every admitted request that has none (at most wave_paths paths, the first
+// request always); otherwise one more sample for every request whose first
+// sample is out, then leftover room as extra samples (at most bulk_paths).
+// *enqueued tells whether there was anything to enqueue.
+int enqueue(cn_server* sv, uint32_t lane, bool first, bool* enqueued) {
+    *enqueued = false;
+    cn_renderer* r = sv->lanes[lane];
     std::vector<HostJob> jobs;
     std::vector<uint64_t> ids;
     std::vector<uint32_t> counts;
-    uint64_t room = sv->wave_paths;
+    uint64_t room = first ? sv->wave_paths : sv->bulk_paths;
     auto add = [&](Request& q, uint32_t n) -> int {
         HostJob job;
         if (int e = film_job(r, q.desc.scene, q.film, q.desc.camera, n, &job)) return e;
@@ -201,34 +238,41 @@ int enqueue(cn_server* sv) {
         counts.push_back(n);
         return CN_OK;
     };
-    // First frames, then one more sample for everyone, in admission order.
-    for (int pass = 0; pass < 2; ++pass) {
-        for (uint64_t id : sv->active) {
-            Request& q = sv->requests[id];
-            if (!wants_more(q) || jobs.size() >= MAX_JOBS) continue;
-            if ((pass == 0) != (q.scheduled == 0)) continue;
-            const uint64_t px = uint64_t(q.desc.width) * q.desc.height;
-            if (px > room && !jobs.empty()) continue;
-            if (int e = add(q, 1)) return e;
-            room = px > room ? 0 : room - px;
-        }
+    for (uint64_t id : sv->active) {
+        Request& q = sv->requests[id];
+        if (!wants_more(q) || jobs.size() >= MAX_JOBS) continue;
+        if (first != (q.scheduled == 0)) continue;
+        const uint64_t px = uint64_t(q.desc.width) * q.desc.height;
+        if (px > room && !jobs.empty()) continue;
+        if (int e = add(q, 1)) return e;
+        room = px > room ? 0 : room - px;
     }
     if (jobs.empty()) return CN_OK;
-    // Leftover room: more samples of the same requests, in order.
-    for (size_t k = 0; k < jobs.size(); ++k) {
-        Request& q = sv->requests[ids[k]];
-        const uint64_t px = uint64_t(q.desc.width) * q.desc.height;
-        const uint64_t extra = std::min<uint64_t>({uint64_t(q.desc.spp - q.scheduled - 1), room / px, 65534u});
-        jobs[k].samples += uint32_t(extra);
-        counts[k] += uint32_t(extra);
-        room -= extra * px;
+    if (!first) {
+        // Leftover room: more samples of the same requests, in order.
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            Request& q = sv->requests[ids[k]];
+            const uint64_t px = uint64_t(q.desc.width) * q.desc.height;
+            const uint64_t extra = std::min<uint64_t>({uint64_t(q.desc.spp - q.scheduled - 1), room / px, 65534u});
+            jobs[k].samples += uint32_t(extra);
+            counts[k] += uint32_t(extra);
+            room -= extra * px;
+        }
     }
-    hipEvent_t previous = sv->inflight.empty() ? nullptr : sv->inflight.back().event;
-    if (int e = enqueue_wave(r, jobs, true, nullptr, previous)) return e;
-    sv->lane_next = (sv->lane_next + 1) % LANES;
+    // The accumulate waits for the in-flight waves that fold into any of
+    // this wave's films.
+    std::vector<hipEvent_t> after;
+    for (uint64_t id : ids) {
+        const uint64_t last = sv->requests[id].last_wave;
+        for (const InFlight& w : sv->inflight)
+            if (w.serial == last && std::find(after.begin(), after.end(), w.event) == after.end())
+                after.push_back(w.event);
+    }
+    if (int e = enqueue_wave(r, jobs, true, nullptr, after.data(), uint32_t(after.size()))) return e;
     InFlight w;
-    w.ring = sv->ring_next;
-    sv->ring_next = (sv->ring_next + 1) % NOISE_RING;
+    w.serial = ++sv->wave_serial;
+    w.lane = lane;
+    w.ring = lane;
     double* dev_partials = sv->partials.ptr + size_t(w.ring) * MAX_JOBS * NOISE_BLOCKS;
     double* host_partials = sv->host_partials + size_t(w.ring) * MAX_JOBS * NOISE_BLOCKS;
     int noise_slots = 0;
@@ -236,6 +280,7 @@ int enqueue(cn_server* sv) {
         Request& q = sv->requests[ids[k]];
         q.scheduled += counts[k];
         q.film->samples = q.scheduled;
+        q.last_wave = w.serial;
         w.paths += uint64_t(jobs[k].width) * jobs[k].height * counts[k];
         int slot = -1;
         if (q.desc.rel_mse > 0.0f && q.scheduled >= 2) {
@@ -258,8 +303,10 @@ int enqueue(cn_server* sv) {
     w.event = sv->free_events.back();
     sv->free_events.pop_back();
     CN_HIP(hipEventRecord(w.event, r->stream));
+    sv->lane_busy[lane] = true;
     sv->inflight.push_back(std::move(w));
     sv->progress += 1;
+    *enqueued = true;
     return CN_OK;
 }
 
@@ -275,20 +322,22 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
     sv->r = renderer;
     sv->budget = hbm_budget_bytes;
     sv->wave_paths = std::min(wave_paths == 0 ? (1u << 24) : wave_paths, renderer->capacity);
+    sv->bulk_paths = uint32_t(std::min<uint64_t>(uint64_t(sv->wave_paths) * BULK_FACTOR, renderer->capacity));
     sv->t0 = std::chrono::steady_clock::now();
     // The lanes copy the caller's renderer's depth, light sampling and launch
-    // shape; each holds a pool of wave_paths paths, allocated here.
+    // shape; each holds a pool of its waves' size, allocated here.
     auto lanes_up = [&]() -> int {
         cn_launch_shape shape;
         if (int e = cn_renderer_get_shape(renderer, &shape)) return e;
         for (uint32_t k = 0; k < LANES; ++k) {
+            const uint32_t paths = k == 0 ? sv->wave_paths : sv->bulk_paths;
             cn_renderer* lane = nullptr;
-            if (int e = cn_renderer_create(renderer->ctx, sv->wave_paths, renderer->max_bounces,
-                                           renderer->light_sampling, &lane))
+            if (int e = cn_renderer_create(renderer->ctx, paths, renderer->max_bounces, renderer->light_sampling,
+                                           &lane))
                 return e;
             sv->lanes.push_back(lane);
             if (int e = cn_renderer_set_shape(lane, &shape)) return e;
-            if (int e = cn_renderer_reserve(lane, sv->wave_paths, 1)) return e;
+            if (int e = cn_renderer_reserve(lane, paths, 1)) return e;
         }
         CN_HIP(sv->partials.alloc(size_t(NOISE_RING) * MAX_JOBS * NOISE_BLOCKS));
         CN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sv->host_partials),
@@ -352,14 +401,25 @@ int cn_server_submit(cn_server* sv, const cn_request_desc* desc, uint64_t* id) {
 int cn_server_step(cn_server* sv, uint32_t* busy) {
     if (!sv) return fail(CN_E_INVALID, "cn_server_step: null server");
     CN_HIP(hipSetDevice(sv->r->ctx->device));
+    uint32_t retired = 0;
+    if (int e = retire_ready(sv, &retired)) return e;
     if (int e = admit(sv)) return e;
-    const size_t before = sv->inflight.size();
-    if (int e = enqueue(sv)) return e;
-    const bool enqueued = sv->inflight.size() > before;
-    // Keep at most LANES waves in flight; when there was nothing new to
-    // enqueue, retire so finishing requests report.
-    if (sv->inflight.size() >= LANES || (!enqueued && !sv->inflight.empty())) {
-        if (int e = retire_one(sv)) return e;
+    bool enqueued = false;
+    for (uint32_t lane = 0; lane < LANES; ++lane) {
+        if (sv->lane_busy[lane]) continue;
+        bool more = false;
+        if (int e = enqueue(sv, lane, lane == 0, &more)) return e;
+        enqueued = enqueued || more;
+    }
+    // Nothing new and nothing retired: wait a little for the next wave to
+    // complete, so callers can loop on step without spinning, while new
+    // arrivals still reach the free first-frame lane within a millisecond.
+    if (!enqueued && retired == 0 && !sv->inflight.empty()) {
+        const auto until = std::chrono::steady_clock::now() + std::chrono::milliseconds(1);
+        while (retired == 0 && std::chrono::steady_clock::now() < until) {
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+            if (int e = retire_ready(sv, &retired)) return e;
+        }
     }
     if (busy) *busy = uint32_t(sv->queue.size() + sv->active.size() + sv->inflight.size());
     return CN_OK;
@@ -371,7 +431,7 @@ int cn_server_drain(cn_server* sv) {
     while (busy) {
         const uint64_t before = sv->progress;
         if (int e = cn_server_step(sv, &busy)) return e;
-        if (busy && sv->progress == before) {
+        if (busy && sv->progress == before && sv->inflight.empty()) {
             // Only queued requests remain and none fits: finished requests'
             // films hold the budget until their results are taken.
             return fail(CN_E_ALLOC, "%u queued request(s) wait for budget held by finished requests' films",

@@ -2,8 +2,9 @@
 
 ``Server`` wraps the library's request loop (cn_server_*): requests are
 admitted while their films fit an HBM budget and interleaved into shared
-waves at sample granularity, up to three waves tracing at once on the
-server's own streams. ``poisson_requests``
+waves at sample granularity — first frames on a lane of their own, bulk
+samples in larger waves on two more — with up to three waves tracing at
+once on the server's own streams. ``poisson_requests``
 draws a seeded open-loop request stream; ``run_served`` replays it against
 a server in real time and ``run_plain`` against the plain form — one
 request at a time, one sample per wave from a reference-sized pool, as the

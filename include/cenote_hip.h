@@ -458,14 +458,15 @@ int cn_render_batch(cn_renderer* renderer, const cn_view_job* views, uint32_t co
  *
  * A request loop that admits progressive render requests while their films
  * fit an HBM budget, keeps scenes resident, and interleaves the admitted
- * requests into shared waves at sample granularity: each step builds one
- * wave in which every request without a sample gets its first one, every
- * other unfinished request one more, and leftover pool room more samples in
- * admission order. Successive waves go to three lanes of the server's own
- * (renderers with their own streams and path pools, copying the given
- * renderer's depth, light sampling and launch shape), so up to three waves
- * trace at once; each wave's accumulate waits for the previous wave's. A
- * request's film receives its samples in index order, so its result is
+ * requests into shared waves at sample granularity. Waves run on three lanes
+ * of the server's own (renderers with their own streams and path pools,
+ * copying the given renderer's depth, light sampling and launch shape), one
+ * wave in flight per lane. The first-frame lane takes, as soon as it is
+ * free, one sample of every admitted request that has none yet; the two
+ * bulk lanes take waves of one more sample for every other unfinished
+ * request plus leftover room as more samples in admission order. A wave's
+ * accumulate waits for the earlier waves that fold into its films, so a
+ * request's film receives its samples in index order and its result is
  * bitwise the same alone as in any mix. Times are seconds on the server's
  * steady clock. */
 typedef struct cn_server cn_server;
@@ -507,17 +508,19 @@ typedef struct cn_server_stats {
     uint64_t hbm_used_bytes;      /* films held by requests */
 } cn_server_stats;
 
-/* wave_paths: most paths per wave (0 = 2^24; at most the renderer's
- * capacity; smaller waves return first frames sooner). Each lane allocates
- * a pool of that many paths (228 bytes each) at creation. */
+/* wave_paths: most paths per first-frame wave (0 = 2^24; at most the
+ * renderer's capacity; smaller waves return first frames sooner); bulk waves
+ * hold up to 4 x wave_paths (at most the renderer's capacity). Each lane
+ * allocates a pool of its waves' size (228 bytes per path) at creation. */
 int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t wave_paths,
                      cn_server** out);
 int cn_server_destroy(cn_server* server);
 int cn_server_clock(cn_server* server, double* seconds);
 int cn_server_submit(cn_server* server, const cn_request_desc* request, uint64_t* id);
-/* One scheduling step: admit, enqueue one wave, retire the oldest wave when
- * two are in flight (or when there was nothing to enqueue). `busy` = queued
- * + active requests + waves in flight (0: idle). */
+/* One scheduling step: retire the waves that have completed, admit, and
+ * enqueue a wave on every free lane that has work; when there was nothing to
+ * do, wait up to a millisecond for a wave to complete. `busy` = queued +
+ * active requests + waves in flight (0: idle). */
 int cn_server_step(cn_server* server, uint32_t* busy);
 /* Step until idle. Fails with CN_E_ALLOC, instead of waiting forever, when
  * queued requests cannot be admitted because finished requests' films

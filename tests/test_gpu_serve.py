@@ -130,7 +130,11 @@ def test_budget_admission_holds_requests_back(ctx, scenes):
     r.close()
 
 
-def test_a_late_request_gets_its_first_frame_within_two_waves(ctx, scenes):
+def test_a_late_request_gets_its_first_frame_without_queueing(ctx, scenes):
+    """A request arriving while a long one keeps both bulk lanes busy gets
+    its first frame from the first-frame lane at once: while the long
+    request still runs, and before the bulk lanes turn over more than twice
+    each."""
     data, handles = scenes
     r = cn.Renderer(ctx, max_bounces=8)
     sv = serve.Server(r, 1 << 30, wave_paths=64 * 64 * 4)
@@ -138,12 +142,14 @@ def test_a_late_request_gets_its_first_frame_within_two_waves(ctx, scenes):
     for _ in range(4):
         sv.step()
     late = sv.submit(handles["chart"], data["chart"].camera, 32, 32, 2, max_bounces=8)
+    waves = sv.stats().waves
     steps = 0
     while sv.status(late).first_frame_seconds < 0:
         sv.step()
         steps += 1
-        assert steps <= 3
+        assert steps <= 1000
     assert sv.status(long_id).state == _abi.REQUEST_ACTIVE
+    assert sv.stats().waves - waves <= 5  # the late wave and at most two per bulk lane
     sv.drain()
     sv.close()
     r.close()
