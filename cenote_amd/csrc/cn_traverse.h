@@ -187,8 +187,7 @@ __device__ __forceinline__ SlabRay make_slab(v3 o, v3 d) {
 // Byte c of a plane word as a float (one v_cvt_f32_ubyte).
 __device__ __forceinline__ float plane(uint32_t word, int c) { return float((word >> (8 * c)) & 0xffu); }
 
-// One compare-exchange of a stable sort by key: adjacent positions only,
-// swapping on strictly greater, so equal keys keep their slot order.
+// One compare-exchange by key, swapping on strictly greater.
 __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, uint32_t& cb) {
     const bool swap = ka > kb;
     const float k = swap ? kb : ka;
@@ -201,9 +200,11 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 
 // Box-test the four children of one node against [0, bound]; push the hit
 // ones farthest first and return the nearest (or the next stack entry).
-// Hit children are ordered by entry distance, ties in slot order: an
-// odd-even transposition network (six adjacent compare-exchanges, no
-// branches), misses keyed +inf so they sort last.
+// Hit children are ordered by entry distance with the optimal four-input
+// sorting network (five compare-exchanges in three rounds, no branches),
+// misses keyed +inf so they sort last. Equal keys may leave slot order,
+// which changes only the visiting order: the closest hit, visibility and
+// transmittance do not depend on it.
 // A plane at origin + q * step meets the ray at t = q * (step / d) +
 // (origin - o) / d: one fma per plane after six per-node products. The
 // step is positive, so on each axis the low plane is met first exactly
@@ -240,9 +241,8 @@ __device__ __forceinline__ uint32_t visit(const BvhNode4Q* __restrict__ nodes, u
     }
     order_pair(key[0], ref[0], key[1], ref[1]);
     order_pair(key[2], ref[2], key[3], ref[3]);
-    order_pair(key[1], ref[1], key[2], ref[2]);
-    order_pair(key[0], ref[0], key[1], ref[1]);
-    order_pair(key[2], ref[2], key[3], ref[3]);
+    order_pair(key[0], ref[0], key[2], ref[2]);
+    order_pair(key[1], ref[1], key[3], ref[3]);
     order_pair(key[1], ref[1], key[2], ref[2]);
     if (nh == 0) return st.pop_or_done();
     st.push_sorted(ref[3], ref[2], ref[1], nh);
