@@ -9,9 +9,10 @@
 // frames: a wave of one sample for every admitted request that has none
 // yet, at most wave_paths paths, launched as soon as the lane is free so
 // that a new request's first frame never queues behind bulk work. The other
-// lanes carry bulk waves of up to BULK_FACTOR x wave_paths paths: one more
-// sample for every request whose first sample is out, then leftover room as
-// extra samples in admission order. Bulk waves are large, so their sparse
+// lanes carry bulk waves — 4 x wave_paths paths while requests wait for a
+// first frame, 8 x otherwise — of one more sample for every request whose
+// first sample is out, then leftover room as extra samples in admission
+// order. Bulk waves are large, so their sparse
 // deep bounces are a small share of them, and the lanes overlap one wave's
 // deep bounces with the others' full ones. A wave's accumulate waits for the
 // earlier waves that fold into any of its films (events across lanes), so
@@ -38,7 +39,12 @@ namespace {
 
 constexpr uint32_t NOISE_BLOCKS = 64;  // partial sums per noise estimate
 constexpr uint32_t LANES = 3;          // lane 0: first frames; lanes 1 and 2: bulk waves
-constexpr uint32_t BULK_FACTOR = 4;    // a bulk wave holds up to this many times wave_paths
+// A bulk wave holds up to BULK_FACTOR x wave_paths paths while requests wait
+// for their first frame (smaller bulk waves leave the first-frame lane more
+// of the GPU), and up to BULK_FACTOR_IDLE x wave_paths otherwise (larger
+// waves spend less of their time in sparse deep bounces).
+constexpr uint32_t BULK_FACTOR = 4;
+constexpr uint32_t BULK_FACTOR_IDLE = 8;
 
 size_t film_bytes(uint32_t width, uint32_t height, bool noise) {
     // beauty, albedo, normal sums (16 B each) + depth (4 B) [+ moments (4 B)]
@@ -82,8 +88,9 @@ struct cn_server {
     bool lane_busy[LANES] = {};
     uint64_t wave_serial = 0;
     uint64_t budget = 0;
-    uint32_t wave_paths = 0;  // first-frame waves
-    uint32_t bulk_paths = 0;  // bulk waves
+    uint32_t wave_paths = 0;       // first-frame waves
+    uint32_t bulk_paths = 0;       // bulk waves while first frames are pending
+    uint32_t bulk_paths_idle = 0;  // bulk waves otherwise
     std::chrono::steady_clock::time_point t0;
     std::map<uint64_t, Request> requests;
     std::deque<uint64_t> queue;    // waiting for admission, FIFO
@@ -226,7 +233,9 @@ int enqueue(cn_server* sv, uint32_t lane, bool first, bool* enqueued) {
     std::vector<HostJob> jobs;
     std::vector<uint64_t> ids;
     std::vector<uint32_t> counts;
-    uint64_t room = first ? sv->wave_paths : sv->bulk_paths;
+    bool waiting = !sv->queue.empty();
+    for (uint64_t id : sv->active) waiting = waiting || sv->requests[id].scheduled == 0;
+    uint64_t room = first ? sv->wave_paths : (waiting ? sv->bulk_paths : sv->bulk_paths_idle);
     auto add = [&](Request& q, uint32_t n) -> int {
         HostJob job;
         if (int e = film_job(r, q.desc.scene, q.film, q.desc.camera, n, &job)) return e;
@@ -323,6 +332,8 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
     sv->budget = hbm_budget_bytes;
     sv->wave_paths = std::min(wave_paths == 0 ? (1u << 24) : wave_paths, renderer->capacity);
     sv->bulk_paths = uint32_t(std::min<uint64_t>(uint64_t(sv->wave_paths) * BULK_FACTOR, renderer->capacity));
+    sv->bulk_paths_idle =
+        uint32_t(std::min<uint64_t>(uint64_t(sv->wave_paths) * BULK_FACTOR_IDLE, renderer->capacity));
     sv->t0 = std::chrono::steady_clock::now();
     // The lanes copy the caller's renderer's depth, light sampling and launch
     // shape; each holds a pool of its waves' size, allocated here.
@@ -330,9 +341,10 @@ int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t 
         cn_launch_shape shape;
         if (int e = cn_renderer_get_shape(renderer, &shape)) return e;
         for (uint32_t k = 0; k < LANES; ++k) {
-            const uint32_t paths = k == 0 ? sv->wave_paths : sv->bulk_paths;
+            const uint32_t paths = k == 0 ? sv->wave_paths : sv->bulk_paths_idle;
+            const uint32_t capacity = paths;
             cn_renderer* lane = nullptr;
-            if (int e = cn_renderer_create(renderer->ctx, paths, renderer->max_bounces, renderer->light_sampling,
+            if (int e = cn_renderer_create(renderer->ctx, capacity, renderer->max_bounces, renderer->light_sampling,
                                            &lane))
                 return e;
             sv->lanes.push_back(lane);

@@ -510,8 +510,10 @@ typedef struct cn_server_stats {
 
 /* wave_paths: most paths per first-frame wave (0 = 2^24; at most the
  * renderer's capacity; smaller waves return first frames sooner); bulk waves
- * hold up to 4 x wave_paths (at most the renderer's capacity). Each lane
- * allocates a pool of its waves' size (228 bytes per path) at creation. */
+ * hold up to 4 x wave_paths while requests wait for their first frame and
+ * 8 x wave_paths otherwise (at most the renderer's capacity). Each lane
+ * allocates a pool of its largest waves' size (228 bytes per path) at
+ * creation. */
 int cn_server_create(cn_renderer* renderer, uint64_t hbm_budget_bytes, uint32_t wave_paths,
                      cn_server** out);
 int cn_server_destroy(cn_server* server);
