@@ -38,9 +38,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mpaths/s + p50/p99 first-frame latency, served vs plain, at relMSE 1e-3"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("raygen", "trace", "shade_miss", "shade_surface", "accumulate")
+KERNELS = ("raygen", "trace", "shade_miss", "shade_surface", "accumulate", "tail")
 KERNEL_SYMBOLS = {"raygen": "k_raygen", "trace": "k_trace", "shade_miss": "k_shade_miss",
-                  "shade_surface": "k_shade_surface", "accumulate": "k_accumulate"}
+                  "shade_surface": "k_shade_surface", "accumulate": "k_accumulate", "tail": "k_tail"}
 
 # Algorithmic bytes per unit of work, in the reference's record layout
 # (crates/cenote/src/wavefront.rs:211-356, shaders/pathstate.slang): a build
@@ -131,24 +131,35 @@ def waves_per_frame(width, height, spp, capacity):
     return -(-spp // per)
 
 
-def timed_launches(name, steps, capacity):
+def tail_split(depth, tail):
+    """The bounce the tail launch takes over after (its trace is the last
+    wavefront trace), or `depth` when every bounce runs as stages."""
+    return tail if 0 < tail < depth else depth
+
+
+def timed_launches(name, steps, capacity, tail=0):
     """Dispatches of each stage kernel in the timed region of `steps` frames."""
     from cenote_amd import workloads
 
     cfg = workloads.CONFIGS[name]
     waves = waves_per_frame(cfg["width"], cfg["height"], cfg["spp"], capacity) * steps
-    rounds = waves * cfg["max_bounces"]
-    return {"raygen": waves, "trace": rounds, "shade_miss": rounds, "shade_surface": rounds, "accumulate": waves}
+    depth = cfg["max_bounces"]
+    t = tail_split(depth, tail)
+    traced = waves * min(depth, t + 1)
+    return {"raygen": waves, "trace": traced, "shade_miss": waves * t, "shade_surface": waves * t,
+            "accumulate": waves, "tail": waves if t < depth else 0}
 
 
-def shape_key(name, capacity, width=4):
+def shape_key(name, capacity, width=4, tail=0):
     """The run shape a measured-traffic entry belongs to."""
     from cenote_amd import workloads
 
     cfg = workloads.CONFIGS[name]
     spw = min(cfg["spp"], max(1, capacity // (cfg["width"] * cfg["height"])))
+    t = tail_split(cfg["max_bounces"], tail)
     return (f"{name} {cfg['width']}x{cfg['height']} spp {cfg['spp']} depth {cfg['max_bounces']} "
-            f"pool {capacity} samples/wave {spw} bvh {width}-wide")
+            f"pool {capacity} samples/wave {spw} bvh {width}-wide"
+            + (f" tail after bounce {t}" if t < cfg["max_bounces"] else ""))
 
 
 def load_traffic(key):
@@ -161,27 +172,41 @@ def load_traffic(key):
     return entry if isinstance(entry, dict) else None
 
 
-def algorithmic_bytes(counts, traversal, info, pixels, aov, width=4):
-    """Per-kernel algorithmic bytes of the counted waves; a node fetch counts
-    its record size (64 bytes 4-wide, 128 bytes 8-wide)."""
+def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
+    """Per-kernel algorithmic bytes of the counted waves, bounce by bounce; a
+    node fetch counts its record size (64 bytes 4-wide, 128 bytes 8-wide).
+    With a tail after bounce t, trace holds bounces 0..t, shade_miss and
+    shade_surface bounces 0..t-1, and the tail kernel every later trace and
+    all shading from bounce t on. `by_bounce[b]` = ray nodes, ray triangles,
+    shadow nodes, shadow triangles of bounce b's trace launch (whose shadow
+    rays are bounce b-1's connections)."""
     node = 128 if width == 8 else info.bvh_node_bytes
     tri = info.bvh_triangle_bytes
-    rays = int(counts["rays"].sum())
-    hits = int(counts["hits"].sum())
-    misses = int(counts["misses"].sum())
-    shadows = int(counts["shadows"].sum())
-    return {
+    depth = len(counts["rays"])
+    traced, surface, miss = [], [], []
+    for b in range(depth):
+        rn, rt, sn, st = (int(v) for v in by_bounce[b])
+        connections = int(counts["shadows"][b - 1]) if b > 0 else 0
+        # intersect fused with the previous bounce's trace_shadow
+        traced.append(BYTES["ray"] * int(counts["rays"][b]) + node * rn + tri * rt
+                      + BYTES["shadow"] * connections + node * sn + tri * st)
+        surface.append(BYTES["hit"] * int(counts["hits"][b]))
+        miss.append(BYTES["miss"] * int(counts["misses"][b]))
+    t = tail_split(depth, tail)
+    out = {
         "raygen": pixels * 72,
         "accumulate": pixels * (BYTES["camera_path_aov"] if aov else 48),
-        # intersect fused with the previous bounce's trace_shadow
-        "trace": BYTES["ray"] * rays + node * traversal["ray_nodes"] + tri * traversal["ray_triangles"]
-        + BYTES["shadow"] * shadows + node * traversal["shadow_nodes"] + tri * traversal["shadow_triangles"],
-        "shade_surface": BYTES["hit"] * hits,
-        "shade_miss": BYTES["miss"] * misses,
-    }, dict(rays=rays, hits=hits, misses=misses, shadows=shadows)
+        "trace": sum(traced[: t + 1]),
+        "shade_surface": sum(surface[:t]),
+        "shade_miss": sum(miss[:t]),
+    }
+    if t < depth:
+        out["tail"] = sum(traced[t + 1:]) + sum(surface[t:]) + sum(miss[t:])
+    totals = {k: int(counts[k].sum()) for k in ("rays", "hits", "misses", "shadows")}
+    return out, totals
 
 
-def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bvh_width=0):
+def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bvh_width=0, tail_bounce=None):
     """`steps` full frames of workload `name`, each a reset film and the
     workload's spp, stream-ordered with one synchronize at the end."""
     import cenote_amd as cn
@@ -200,6 +225,8 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
     renderer = cn.Renderer(ctx, capacity=capacity, max_bounces=depth)
     if bvh_width:
         renderer.shape = {"bvh_width": bvh_width}
+    if tail_bounce is not None:
+        renderer.shape = {"tail_bounce": tail_bounce}
     # the tree the traversal walks (a scene without an 8-wide tree is walked 4-wide)
     walked = 8 if renderer.shape["bvh_width"] == 8 and info.bvh8_node_count else 4
     film = cn.Film(ctx, width, height)
@@ -220,7 +247,9 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
         counts = c if counts is None else {k: counts[k] + c[k] for k in c}
         done += n
     traversal = renderer.traversal_counts()
+    by_bounce = renderer.traversal_counts_by_bounce()
     renderer.set_counters(False)
+    tail = renderer.shape["tail_bounce"]
 
     def frames(k):
         for _ in range(k):
@@ -237,7 +266,7 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
     times = renderer.kernel_times()
     renderer.set_profiling(False)
 
-    frame_bytes, totals = algorithmic_bytes(counts, traversal, info, pixels * spp, aov, walked)
+    frame_bytes, totals = algorithmic_bytes(counts, by_bounce, info, pixels * spp, aov, walked, tail)
     value = pixels * spp * steps / elapsed / 1e6
     grays = (totals["rays"] + totals["shadows"]) * steps / elapsed / 1e9
     kernel_ms = {k: round(v[0] / steps, 4) for k, v in times.items()}
@@ -247,7 +276,7 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
     dom_ms, dom_launches = times[dominant]
     achieved = frame_bytes[dominant] * steps / (dom_ms * 1e-3) / 1e9
     avg_ms = dom_ms / max(1, dom_launches)
-    key = shape_key(name, renderer.capacity, walked)
+    key = shape_key(name, renderer.capacity, walked, tail)
     measured = load_traffic(key)
     traffic = None if measured is None else measured.get(dominant)
     hit_share = totals["hits"] and float(counts["hits"][0]) / float(counts["hits"][0] + counts["misses"][0])
@@ -301,6 +330,7 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
             "triangles": data.triangle_count, "pixels": pixels, "spp": spp, "path_pool": renderer.capacity,
             "samples_per_wave": per_wave,
             "bvh_width": walked,
+            "tail_after_bounce": tail_split(depth, tail) if tail_split(depth, tail) < depth else None,
         },
     }
     if cpu:
@@ -556,7 +586,7 @@ def _short(kernel_name):
     return _base_name(kernel_name).replace("void ", "").replace("cn::", "").strip()
 
 
-def reduce_profile(directory, name, steps, capacity):
+def reduce_profile(directory, name, steps, capacity, tail=0):
     """The timed region's dispatches of each stage kernel in a rocprofv3
     output directory of `bench.py --workload name --steps steps --warmup W
     --main-only`: per-launch average duration (kernel trace) or counter
@@ -564,7 +594,7 @@ def reduce_profile(directory, name, steps, capacity):
     frames come last); the first timed frame's trace dispatches one by one
     (its bounces, in order); and every other kernel (scene build, the
     calibration kernels) averaged over all its dispatches."""
-    launches = timed_launches(name, steps, capacity)
+    launches = timed_launches(name, steps, capacity, tail)
     out, others, by_bounce = {}, {}, []
     per_dispatch = {}  # dispatch id -> {"kernel", "stage", "us", counters...}
     for r in _csv_rows(directory, "kernel_trace.csv"):
@@ -588,7 +618,7 @@ def reduce_profile(directory, name, steps, capacity):
                 for k in keys}
 
     for stage, ds in stages.items():
-        timed = ds[-launches[stage]:] if len(ds) >= launches[stage] else []
+        timed = ds[-launches[stage]:] if 0 < launches[stage] <= len(ds) else []
         out[stage] = {"dispatches_all": len(ds), "dispatches_timed": len(timed), **(averages(timed) if timed else {})}
         if stage == "trace" and timed:
             per_frame = launches["trace"] // max(1, steps)
@@ -610,7 +640,7 @@ def _counter_per_launch(directory, counter):
     return [per[k] for k in sorted(per)]
 
 
-def reduce_traffic(prof, name, steps, capacity, width=4):
+def reduce_traffic(prof, name, steps, capacity, width=4, tail=0):
     """Measured HBM bytes per launch of each stage kernel, from the FETCH_SIZE
     and WRITE_SIZE runs of `bench.py --workload name --steps steps --main-only`
     (prof/pmc_FETCH_SIZE_<name>, prof/pmc_WRITE_SIZE_<name>), each counter
@@ -622,8 +652,8 @@ def reduce_traffic(prof, name, steps, capacity, width=4):
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         copies = [v for k, v in _counter_per_launch(prof / f"pmc_{counter}_copy", counter) if "copyBuffer" in k]
         factors[counter] = (1 << 30) / (sum(copies) / len(copies) * 1024.0)
-    fetch = reduce_profile(prof / f"pmc_FETCH_SIZE_{name}", name, steps, capacity)["kernels"]
-    write = reduce_profile(prof / f"pmc_WRITE_SIZE_{name}", name, steps, capacity)["kernels"]
+    fetch = reduce_profile(prof / f"pmc_FETCH_SIZE_{name}", name, steps, capacity, tail)["kernels"]
+    write = reduce_profile(prof / f"pmc_WRITE_SIZE_{name}", name, steps, capacity, tail)["kernels"]
     entry = {}
     for stage in KERNEL_SYMBOLS:
         f_kib, w_kib = fetch[stage].get("FETCH_SIZE_avg"), write[stage].get("WRITE_SIZE_avg")
@@ -635,7 +665,7 @@ def reduce_traffic(prof, name, steps, capacity, width=4):
     entry["_factors"] = {k: round(v, 4) for k, v in factors.items()}
     entry["_source"] = (f"rocprofv3 --pmc FETCH_SIZE | WRITE_SIZE -- python3 bench.py --workload {name} --steps {steps} "
                         f"--warmup 1 --main-only --no-cpu-baseline; calibrated by bench.py --copy-calibration")
-    return {shape_key(name, capacity, width): entry}
+    return {shape_key(name, capacity, width, tail): entry}
 
 
 # --- main ----------------------------------------------------------------------
@@ -655,6 +685,9 @@ def main():
     ap.add_argument("--capacity", type=int, default=0, help="path-pool capacity (0: the library's default)")
     ap.add_argument("--bvh-width", type=int, default=0, choices=[0, 4, 8],
                     help="the tree the traversal walks (0: the library's default)")
+    ap.add_argument("--tail-bounce", type=int, default=None,
+                    help="the bounce after whose trace one launch finishes every live path (0: none; "
+                         "default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -673,15 +706,17 @@ def main():
         from cenote_amd import _abi
 
         cap = args.capacity or _abi.POOL_CAPACITY
-        print(json.dumps(reduce_traffic(args.reduce_traffic, args.workload, args.steps, cap, args.bvh_width or 4),
+        tail = _abi.DEFAULT_TAIL_BOUNCE if args.tail_bounce is None else args.tail_bounce
+        print(json.dumps(reduce_traffic(args.reduce_traffic, args.workload, args.steps, cap, args.bvh_width or 4, tail),
                          indent=1))
         return
     if args.reduce:
         from cenote_amd import _abi
 
         cap = args.capacity or _abi.POOL_CAPACITY
-        print(json.dumps({"shape_key": shape_key(args.workload, cap, args.bvh_width or 4), "steps": args.steps,
-                          **reduce_profile(args.reduce, args.workload, args.steps, cap)}, indent=1))
+        tail = _abi.DEFAULT_TAIL_BOUNCE if args.tail_bounce is None else args.tail_bounce
+        print(json.dumps({"shape_key": shape_key(args.workload, cap, args.bvh_width or 4, tail), "steps": args.steps,
+                          **reduce_profile(args.reduce, args.workload, args.steps, cap, tail)}, indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by tile or "
@@ -708,7 +743,7 @@ def main():
 
     data = {args.workload: workloads.build(args.workload)}
     main_run = measure_plain(ctx, args.workload, args.steps, args.warmup, args.capacity, data[args.workload],
-                             bvh_width=args.bvh_width,
+                             bvh_width=args.bvh_width, tail_bounce=args.tail_bounce,
                              cpu=not args.no_cpu_baseline)
     result = {
         "metric": METRIC,
@@ -738,7 +773,8 @@ def main():
             if other == args.workload:
                 continue
             d = workloads.build(other)
-            run = measure_plain(ctx, other, args.steps, args.warmup, args.capacity, d, bvh_width=args.bvh_width)
+            run = measure_plain(ctx, other, args.steps, args.warmup, args.capacity, d, bvh_width=args.bvh_width,
+                                tail_bounce=args.tail_bounce)
             beside[other] = {k: run[k] for k in ("value", "unit", "ms_per_frame", "frames", "grays_per_s",
                                                  "camera_hit_share", "kernel_ms_per_frame", "roofline", "config")}
             if other in ("chart", "cornell"):

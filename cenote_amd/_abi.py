@@ -40,6 +40,7 @@ DIRTY_ALL = 31
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
 POOL_CAPACITY = 1 << 27
+DEFAULT_TAIL_BOUNCE = 5
 STATIC_MAX_PIXELS = 1 << 18
 
 f3 = C.c_float * 3
@@ -205,6 +206,7 @@ class LaunchShape(C.Structure):
         ("shade_blocks", C.c_uint32),
         ("light_blocks", C.c_uint32),
         ("bvh_width", C.c_uint32),
+        ("tail_bounce", C.c_uint32),
     ]
 
 
@@ -306,6 +308,7 @@ SIGNATURES = {
     "cn_renderer_set_counters": [_P, C.c_int],
     "cn_renderer_wave_counts": [_P, C.POINTER(WaveCounts)],
     "cn_renderer_traversal_counts": [_P, C.POINTER(TraversalCounts)],
+    "cn_renderer_traversal_counts_by_bounce": [_P, C.POINTER(C.c_uint64), C.c_uint32],
     "cn_renderer_set_profiling": [_P, C.c_int],
     "cn_renderer_kernel_times": [_P, C.POINTER(KernelTimes)],
     "cn_calibrate_valu": [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],

@@ -3,8 +3,9 @@
 The stage kernels' launch shape — the persistent trace grid, the
 shade_surface workgroup variant (1024/512/256 lanes with the BSDF tables in
 LDS, or 256 lanes reading them through the caches), the shading grid caps
-and the BVH width the traversal walks — does not change any result: every shape renders bitwise the same
-film. It does change speed, and the best shape depends on the scene (BVH
+the BVH width the traversal walks and the bounce from which one tail launch
+finishes every live path — does not change any result: every shape renders
+bitwise the same film. It does change speed, and the best shape depends on the scene (BVH
 depth, material mix) and on how many paths a wave holds. ``tune`` times a
 candidate grid on the caller's own workload and leaves the renderer on the
 fastest shape; ``tune_capacity`` picks the path-pool (and so queue)
@@ -31,11 +32,13 @@ SHADE_VARIANTS = {
 }
 
 
-def candidate_shapes(compute_units: int, trace_per_cu=(16, 32, 64), variants=tuple(SHADE_VARIANTS), widths=(4, 8)):
+def candidate_shapes(compute_units: int, trace_per_cu=(16, 32, 64), variants=tuple(SHADE_VARIANTS), widths=(4, 8),
+                     tails=(0,)):
     """The default search grid: trace workgroups per CU x shade variants x
-    BVH width (the 4- or 8-wide tree a scene carries), with each shade
-    variant's grid capped at 2048 lanes per CU (4096 for the 256-lane ones)."""
-    for tpc, variant, width in itertools.product(trace_per_cu, variants, widths):
+    BVH width (the 4- or 8-wide tree a scene carries) x tail bounce (0: the
+    wavefront stages throughout), with each shade variant's grid capped at
+    2048 lanes per CU (4096 for the 256-lane ones)."""
+    for tpc, variant, width, tail in itertools.product(trace_per_cu, variants, widths, tails):
         block = SHADE_VARIANTS[variant]
         yield {
             "trace_blocks": compute_units * tpc,
@@ -43,6 +46,7 @@ def candidate_shapes(compute_units: int, trace_per_cu=(16, 32, 64), variants=tup
             "shade_blocks": compute_units * ((4096 if block == 256 else 2048) // block),
             "light_blocks": compute_units * 4,
             "bvh_width": width,
+            "tail_bounce": tail,
         }
 
 

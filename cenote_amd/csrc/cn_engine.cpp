@@ -257,6 +257,9 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
     r->last_ranges = uint32_t(plan.size());
 
     const uint32_t per = counters_per_range(r->max_bounces);
+    // The tail launch takes over after this bounce's trace (none: every
+    // bounce as wavefront stages). Counting waves keep every bounce's queues.
+    const uint32_t tail_bounce = (r->shape.tail_bounce == 0 || r->counting) ? UINT32_MAX : r->shape.tail_bounce;
     auto launch_all = [&]() -> int {
         for (size_t k = 0; k < plan.size(); ++k) {
             const Range& rg = plan[k];
@@ -267,8 +270,10 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
             CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), pool, tg, counters,
                                  counter_lines_per_range(r->max_bounces), rg.base, rg.count, r->profiler.get()));
             for (uint32_t b = 0; b < rg.rounds; ++b) {
+                const bool tail = b == tail_bounce;
                 CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, pool, tg, counters, rg.base,
-                                     rg.count, b, r->light_sampling, r->counting, r->profiler.get()));
+                                     rg.count, b, r->light_sampling, r->counting, tail, r->profiler.get()));
+                if (tail) break;
             }
         }
         if (films) {
@@ -287,7 +292,7 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
                                      uint64_t(uintptr_t(tg.radiance)), uint64_t(uintptr_t(pool.origin)),
                                      uint64_t(uintptr_t(pool.stack_overflow)), r->shape.trace_blocks,
                                      r->shape.shade_variant, r->shape.shade_blocks, r->shape.light_blocks,
-                                     r->shape.bvh_width, r->light_sampling};
+                                     r->shape.bvh_width, r->shape.tail_bounce, r->light_sampling};
         for (const Range& rg : plan) {
             key.insert(key.end(), {jobs[rg.g0].scene->serial, uint64_t(rg.g0), uint64_t(rg.g1), rg.base, rg.count,
                                    rg.rounds});
@@ -376,10 +381,11 @@ int cn_renderer_create(cn_context* ctx, uint32_t capacity, uint32_t max_bounces,
     r->shape.shade_blocks = cus * 16;
     r->shape.light_blocks = cus * 4;
     r->shape.bvh_width = 4;
+    r->shape.tail_bounce = CN_DEFAULT_TAIL_BOUNCE;
     CN_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     CN_HIP(r->spill.alloc(size_t(r->shape.trace_blocks) * TRAVERSAL_BLOCK * STACK_SPILL));
-    CN_HIP(r->traversal.alloc(4));
-    CN_HIP(hipMemset(r->traversal.ptr, 0, 4 * sizeof(unsigned long long)));
+    CN_HIP(r->traversal.alloc(4 * TRAVERSAL_ROWS));
+    CN_HIP(hipMemset(r->traversal.ptr, 0, 4 * TRAVERSAL_ROWS * sizeof(unsigned long long)));
     CN_HIP(r->counters.alloc(size_t(MAX_RANGES) * counters_per_range(max_bounces)));
     CN_HIP(hipMemset(r->counters.ptr, 0, r->counters.count * 4));
     CN_HIP(r->jobs.alloc(size_t(JOB_RING) * MAX_JOBS));
@@ -421,6 +427,7 @@ int cn_renderer_get_shape(cn_renderer* r, cn_launch_shape* out) {
     out->shade_blocks = r->shape.shade_blocks;
     out->light_blocks = r->shape.light_blocks;
     out->bvh_width = r->shape.bvh_width;
+    out->tail_bounce = r->shape.tail_bounce;
     return CN_OK;
 }
 
@@ -433,6 +440,7 @@ int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
         return fail(CN_E_INVALID, "unknown shade variant %u", shape->shade_variant);
     if (shape->bvh_width != 4 && shape->bvh_width != 8)
         return fail(CN_E_INVALID, "BVH width %u: 4 or 8", shape->bvh_width);
+    if (shape->tail_bounce > 255) return fail(CN_E_INVALID, "tail bounce %u outside 0..255", shape->tail_bounce);
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
     // The traversal stack's spill area is per lane of the trace grid.
@@ -447,6 +455,7 @@ int cn_renderer_set_shape(cn_renderer* r, const cn_launch_shape* shape) {
     r->shape.shade_blocks = shape->shade_blocks;
     r->shape.light_blocks = shape->light_blocks;
     r->shape.bvh_width = shape->bvh_width;
+    r->shape.tail_bounce = shape->tail_bounce;
     return CN_OK;
 }
 
@@ -471,7 +480,7 @@ int cn_renderer_set_counters(cn_renderer* r, int enabled) {
     r->counting = enabled ? 1 : 0;
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
-    CN_HIP(hipMemset(r->traversal.ptr, 0, 4 * sizeof(unsigned long long)));
+    CN_HIP(hipMemset(r->traversal.ptr, 0, 4 * TRAVERSAL_ROWS * sizeof(unsigned long long)));
     return CN_OK;
 }
 
@@ -494,7 +503,7 @@ int cn_renderer_kernel_times(cn_renderer* r, cn_kernel_times* out) {
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
     r->profiler->collect();
-    static const char* names[K_KINDS] = {"raygen", "trace", "shade_miss", "shade_surface", "accumulate"};
+    static const char* names[K_KINDS] = {"raygen", "trace", "shade_miss", "shade_surface", "accumulate", "tail"};
     std::memset(out, 0, sizeof(*out));
     out->kernels = K_KINDS;
     for (int k = 0; k < K_KINDS; ++k) {
@@ -509,14 +518,25 @@ int cn_renderer_kernel_times(cn_renderer* r, cn_kernel_times* out) {
 
 int cn_renderer_traversal_counts(cn_renderer* r, cn_traversal_counts* out) {
     if (!r || !out) return fail(CN_E_INVALID, "cn_renderer_traversal_counts: null argument");
-    unsigned long long c[4];
+    std::vector<unsigned long long> c(4 * TRAVERSAL_ROWS);
     CN_HIP(hipSetDevice(r->ctx->device));
     CN_HIP(hipStreamSynchronize(r->stream));
-    CN_HIP(hipMemcpy(c, r->traversal.ptr, sizeof(c), hipMemcpyDeviceToHost));
-    out->ray_nodes = c[0];
-    out->ray_triangles = c[1];
-    out->shadow_nodes = c[2];
-    out->shadow_triangles = c[3];
+    CN_HIP(hipMemcpy(c.data(), r->traversal.ptr, c.size() * sizeof(c[0]), hipMemcpyDeviceToHost));
+    unsigned long long sum[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < c.size(); ++i) sum[i % 4] += c[i];
+    out->ray_nodes = sum[0];
+    out->ray_triangles = sum[1];
+    out->shadow_nodes = sum[2];
+    out->shadow_triangles = sum[3];
+    return CN_OK;
+}
+
+int cn_renderer_traversal_counts_by_bounce(cn_renderer* r, uint64_t* out, uint32_t bounces) {
+    if (!r || !out) return fail(CN_E_INVALID, "cn_renderer_traversal_counts_by_bounce: null argument");
+    if (bounces > TRAVERSAL_ROWS) return fail(CN_E_INVALID, "at most %u bounces", TRAVERSAL_ROWS);
+    CN_HIP(hipSetDevice(r->ctx->device));
+    CN_HIP(hipStreamSynchronize(r->stream));
+    CN_HIP(hipMemcpy(out, r->traversal.ptr, size_t(4) * bounces * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return CN_OK;
 }
 

@@ -16,6 +16,8 @@ constexpr uint32_t STACK_MAX = 128;
 constexpr uint32_t STACK_SPILL = STACK_MAX - STACK_LDS;
 // Lanes per workgroup of the traversal kernels (trace, first hits).
 constexpr uint32_t TRAVERSAL_BLOCK = 256;
+// Rows of the traversal counters: one per bounce (depth caps are < 256).
+constexpr uint32_t TRAVERSAL_ROWS = 256;
 
 // shade_surface launch variants: workgroup size and where the BSDF tables live.
 // The _SORTED variants order each workgroup's hits by material class first.
@@ -35,6 +37,8 @@ struct LaunchShape {
     uint32_t shade_blocks;   // most blocks (of the variant's size) for shade_surface
     uint32_t light_blocks;   // blocks of 256 lanes for shade_miss
     uint32_t bvh_width;      // 4 or 8: the tree the traversal kernels walk
+    uint32_t tail_bounce;    // 0: every bounce runs as wavefront stages; b >= 1: after bounce b's
+                             // trace, one tail launch carries each live path to its end
 };
 
 // The film's buffers (render/film.rs:66-127): one wave's sample
@@ -51,8 +55,9 @@ struct FilmView {
 };
 
 // The stage kernels, for per-kernel timing.
-// K_TRACE is intersect fused with the previous bounce's trace_shadow.
-enum KernelKind { K_RAYGEN = 0, K_TRACE, K_SHADE_MISS, K_SHADE_SURFACE, K_ACCUMULATE, K_KINDS };
+// K_TRACE is intersect fused with the previous bounce's trace_shadow;
+// K_TAIL carries the paths alive after the tail bounce's trace to their end.
+enum KernelKind { K_RAYGEN = 0, K_TRACE, K_SHADE_MISS, K_SHADE_SURFACE, K_ACCUMULATE, K_TAIL, K_KINDS };
 
 // Brackets each launch (the renderer records HIP events on the launch
 // stream when profiling is on).
@@ -68,9 +73,11 @@ hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job
                          const WaveTargets& tg, uint32_t* counters, uint32_t counter_lines, uint32_t base,
                          uint32_t count, Profiler* prof);
 
+// One bounce: trace, then shade_miss and shade_surface — or, with `tail`,
+// trace and then the tail launch that finishes every live path.
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t light_sampling, int counting, Profiler* prof);
+                         uint32_t bounce, uint32_t light_sampling, int counting, bool tail, Profiler* prof);
 
 // Fold every job's slots into its film sums (jobs sorted by pixel base;
 // total_pixels is the sum of the jobs' view pixels).

@@ -231,7 +231,8 @@ struct PathPool {
     float4* shadow_contrib;  // xyz, w = target instance (bits)
     uint32_t* shadow_prim;
     uint32_t* stack_overflow;  // per-lane spill area of the traversal stack
-    unsigned long long* traversal_counts;  // 4 counters when enabled
+    unsigned long long* traversal_counts;  // when counting: 4 counters per bounce, [bounce][ray nodes,
+                                           // ray triangles, shadow nodes, shadow triangles]
     uint32_t sub_capacity;     // entries per sub-queue: sub_capacity_for(pool paths)
 };
 

@@ -199,6 +199,11 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
 #ifndef TRACE8_WAVES
 #define TRACE8_WAVES 7
 #endif
+// The tail kernel holds shading's registers and a traversal: three waves
+// per SIMD, as shade_surface.
+#ifndef TAIL_WAVES
+#define TAIL_WAVES 3
+#endif
 
 // COUNT: accumulate node and triangle fetches (the counting pass); the
 // timed form keeps no counters. Eight waves per SIMD (64 VGPRs; LDS 8 x 16
@@ -256,51 +261,60 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         }
     }
     if constexpr (COUNT) {
-        wave_add_u64(&p.traversal_counts[0], ray_nodes);
-        wave_add_u64(&p.traversal_counts[1], ray_tris);
-        wave_add_u64(&p.traversal_counts[2], shadow_nodes);
-        wave_add_u64(&p.traversal_counts[3], shadow_tris);
+        unsigned long long* c = p.traversal_counts + 4 * bounce;  // this launch's bounce row
+        wave_add_u64(&c[0], ray_nodes);
+        wave_add_u64(&c[1], ray_tris);
+        wave_add_u64(&c[2], shadow_nodes);
+        wave_add_u64(&c[3], shadow_tris);
     }
 }
 
 // --- shade_miss (shade_miss.slang:27-75) ----------------------------------------------
+
+// One path whose ray at `bounce` escaped: the environment's radiance, MIS
+// weighted against next-event estimation, lands in the path's slot, and a
+// camera path's AOVs take the miss values.
+__device__ __forceinline__ void miss_path(const SceneView& s, const PathPool& p, const WaveTargets& tg, uint32_t path,
+                                          uint32_t base, uint32_t bounce, uint32_t light_sampling) {
+    const uint32_t pixel = base + path;
+    const v3 direction = xyz(p.direction[path]);
+    const float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
+    float weight = 1.0f;
+    if (thr.w > 0.0f) {
+        if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
+            weight = 0.0f;
+        } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
+            const float neePdf = s.env_select_prob * env_pdf(s, direction);
+            weight = power_heuristic(thr.w, neePdf);
+        }
+    }
+    if (tg.albedo) {
+        const bool cameraRay = thr.w == 0.0f;
+        if (bounce == 0) {
+            // A camera path (thr.w == 0): the slot's first AOV writes.
+            tg.depth[pixel] = 0.0f + __uint_as_float(0x7f800000u);
+            put_pixel(tg.albedo, pixel, S3(1.0f), 0.0f, true);
+            tg.normal[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
+            const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
+            // A path past its first diffuse vertex carries a zero guide.
+            // Adding zeros leaves the sum's bits as they are (it starts
+            // at +0 and never turns -0), so skip that read-modify-write;
+            // any other guide, NaN included, is added as before.
+            if (!(guide.x == 0.0f && guide.y == 0.0f && guide.z == 0.0f)) add_pixel(tg.albedo, pixel, guide, 0.0f);
+        }
+    }
+    const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
+    put_pixel(tg.radiance, pixel, contribution, 1.0f, bounce == 0);
+}
 
 __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
                                                       uint32_t base, uint32_t bounce, uint32_t light_sampling) {
     const uint32_t n = queue_total(counters, bounce, Q_MISS);
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         const uint32_t path = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
-        const uint32_t pixel = base + path;
-        const v3 direction = xyz(p.direction[path]);
-        const float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
-        float weight = 1.0f;
-        if (thr.w > 0.0f) {
-            if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
-                weight = 0.0f;
-            } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
-                const float neePdf = s.env_select_prob * env_pdf(s, direction);
-                weight = power_heuristic(thr.w, neePdf);
-            }
-        }
-        if (tg.albedo) {
-            const bool cameraRay = thr.w == 0.0f;
-            if (bounce == 0) {
-                // A camera path (thr.w == 0): the slot's first AOV writes.
-                tg.depth[pixel] = 0.0f + __uint_as_float(0x7f800000u);
-                put_pixel(tg.albedo, pixel, S3(1.0f), 0.0f, true);
-                tg.normal[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            } else {
-                if (cameraRay) tg.depth[pixel] += __uint_as_float(0x7f800000u);
-                const v3 guide = cameraRay ? S3(1.0f) : xyz(tg.guide[pixel]);
-                // A path past its first diffuse vertex carries a zero guide.
-                // Adding zeros leaves the sum's bits as they are (it starts
-                // at +0 and never turns -0), so skip that read-modify-write;
-                // any other guide, NaN included, is added as before.
-                if (!(guide.x == 0.0f && guide.y == 0.0f && guide.z == 0.0f)) add_pixel(tg.albedo, pixel, guide, 0.0f);
-            }
-        }
-        const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
-        put_pixel(tg.radiance, pixel, contribution, 1.0f, bounce == 0);
+        miss_path(s, p, tg, path, base, bounce, light_sampling);
     }
 }
 
@@ -371,6 +385,209 @@ __device__ __forceinline__ InstanceRecord load_instance(const InstanceRecord* re
     return recs[i];
 }
 
+// The connection a shaded hit sends toward a light (trace_shadow.slang's
+// record): origin and the connection's length, direction, the path's slot,
+// the unshadowed contribution, and the target light's (instance,
+// primitive).
+struct ShadowRecord {
+    v3 origin;
+    float tmax;
+    v3 direction;
+    uint32_t pixel;
+    v3 contrib;
+    uint32_t instance;
+    uint32_t primitive;
+};
+
+// shade_surface for one path's hit at `bounce`: emission into the slot,
+// the next-event connection, and the scattered ray. Every lane that gets
+// as far as the connection calls `send_shadow(push, record)` (push false
+// when there is nothing to connect), so a sink may be a wave-wide claim.
+// Returns true when the path goes on (its next ray and throughput are in
+// the path records).
+template <typename SendShadow>
+__device__ __forceinline__ bool shade_path(const SceneView& s, const PathPool& p, const WaveTargets& tg,
+                                           const float* tables, uint32_t path, uint32_t base, uint32_t bounce,
+                                           uint32_t light_sampling, SendShadow&& send_shadow) {
+    const bool aovOn = tg.albedo != nullptr;
+    bool push_ray = false;
+    const uint32_t pixel = base + path;  // the path's target slot
+    const uint2 key = p.key[path];
+    const uint4 hit = p.hit[path];
+    const float4 dir4 = p.direction[path];
+    const v3 direction = xyz(dir4);
+    const float4 origin4 = p.origin[path];
+    const v3 origin = xyz(origin4);
+    const uint32_t depth_cap = __float_as_uint(origin4.w);
+    const InstanceRecord inst = load_instance(s.instances, hit.x);
+    const Mat material = load_material(s.materials, hit.x);
+    const Spawn spawn = spawn_point(s, inst, hit.y, __uint_as_float(hit.z), __uint_as_float(hit.w), direction);
+    float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
+
+    uint32_t medium = bounce == 0 ? MEDIUM_NONE : (p.state[path] & 0xffffffu);
+    if (medium != MEDIUM_NONE) {
+        const Mat interior = load_material(s.materials, medium);
+        if (interior.transmissionDepth > 0.0f) {
+            const v3 tc = vclamp(interior.transmissionColor, 1e-4f, 1.0f);
+            const v3 sigma = (-V3(logf(tc.x), logf(tc.y), logf(tc.z))) / interior.transmissionDepth;
+            const v3 e = -sigma * length(origin - spawn.position);
+            thr.x *= expf(e.x);
+            thr.y *= expf(e.y);
+            thr.z *= expf(e.z);
+        }
+    }
+    const v3 T = xyz(thr);
+
+    v3 contribution = S3(0.0f);
+    if (inst.light != LIGHT_NONE && spawn.frontFace) {
+        float weight = 1.0f;
+        if (thr.w > 0.0f) {
+            if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
+                weight = 0.0f;
+            } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
+                const v3 delta = spawn.position - origin;
+                const float cosLight = fabsf(dot(spawn.normal, direction));
+                const float neePdf = (1.0f - s.env_select_prob) * s.lights[inst.light + hit.y].pdf *
+                                     dot(delta, delta) / cosLight;
+                weight = power_heuristic(thr.w, neePdf);
+            }
+        }
+        contribution = (T * material.emission) * weight;
+    }
+    // The emission lands now rather than when the path's fate is
+    // known; a terminal path adds its sample count (and an emission
+    // that was not positive) at the end. A slot sees the same sums.
+    const bool lit = any_positive(contribution);
+    if (bounce == 0) put_pixel(tg.radiance, pixel, lit ? contribution : S3(0.0f), 0.0f, true);
+    else if (lit) add_pixel(tg.radiance, pixel, contribution, 0.0f);
+
+    // The next-event candidate (a point on a light, or an
+    // environment direction) depends only on the hit point, so it is
+    // drawn before the closure is resolved; only the candidate stays
+    // live across the resolution.
+    const float envProb = s.env_select_prob;
+    const bool nee = bounce + 1 < depth_cap && light_sampling != CN_LIGHT_SAMPLING_BSDF &&
+                     (s.light_count > 0 || envProb > 0.0f);
+    LightConnection cand{S3(0.0f), 1e30f, S3(0.0f), 0.0f, false, LIGHT_NONE, 0u};
+    bool cand_env = false;
+    if (nee) {
+        const float uLight = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_LIGHT));
+        const f2 uPoint = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_POINT));
+        if (uLight < envProb) {
+            const EnvSample e = env_sample(s, uPoint);
+            cand.direction = e.direction;
+            cand.radiance = e.radiance;
+            cand.pdf = e.pdf;
+            cand_env = true;
+        } else {
+            const float uList = (uLight - envProb) / (1.0f - envProb);
+            cand = sample_lights(s, uList, uPoint, spawn.position);
+        }
+    }
+
+    const v3 shading = spawn.shading;
+    const Frame frame = make_frame(shading);
+    const v3 wo = to_local(frame, -direction);
+    const Closure closure = resolve_closure(tables, material, wo, medium == hit.x);
+
+    v3 guide = S3(0.0f);
+    if (aovOn) {
+        guide = bounce == 0 ? S3(1.0f) : xyz(tg.guide[pixel]);
+        if (bounce == 0) tg.depth[pixel] = 0.0f + length(origin - spawn.position) * dir4.w;
+    }
+    const v3 aovAlbedo = vclamp(closure.aovAlbedo, 0.0f, 1.0f);
+    bool terminal = false;
+
+    if (bounce + 1 >= depth_cap) {
+        terminal = true;
+    } else {
+        // Next-event estimation (shade_surface.slang:363-446), with the
+        // candidate drawn above.
+        bool push_shadow = false;
+        v3 sh_origin = S3(0.0f), sh_contrib = S3(0.0f);
+        if (nee) {
+            const v3 wiLight = to_local(frame, cand.direction);
+            const bool through = dot(cand.direction, spawn.normal) < 0.0f;
+            if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
+                const float neePdf = (cand_env ? envProb : 1.0f - envProb) * cand.pdf;
+                const BsdfEval light = eval_with_pdf(closure, wiLight, wo);
+                const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
+                                         ? power_heuristic(neePdf, light.pdf)
+                                         : 1.0f;
+                const v3 unshadowed =
+                    (((T * cand.radiance) * light.f) * fabsf(wiLight.z)) * (weight / neePdf);
+                sh_origin = through ? spawn.below : spawn.position;
+                if (any_positive(unshadowed)) {
+                    push_shadow = true;
+                    sh_contrib = unshadowed;
+                }
+            }
+        }
+        // The connection goes out before scattering, so its record
+        // is dead through the rest of the shading.
+        send_shadow(push_shadow, ShadowRecord{sh_origin, cand.tmax, cand.direction, pixel, sh_contrib,
+                                              cand.instance, cand.primitive});
+
+        // Scatter (shade_surface.slang:453-489).
+        const float uLobe = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_LOBE));
+        const f2 u = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_DIR));
+        BsdfSample scatter;
+        BsdfEval drawn{S3(0.0f), 0.0f};
+        if (sample_bsdf_direction(closure, wo, uLobe, u, scatter)) drawn = eval_with_pdf(closure, scatter.wi, wo);
+        scatter.pdf = drawn.pdf;
+        const v3 wiWorld = to_world(frame, scatter.wi);
+        v3 scattered = S3(0.0f);
+        if (scatter.pdf > 0.0f && scatter.wi.z != 0.0f &&
+            (scatter.wi.z > 0.0f) == (dot(wiWorld, spawn.normal) > 0.0f)) {
+            scattered = (T * drawn.f) * (fabsf(scatter.wi.z) / scatter.pdf);
+        }
+        float survival = fmaxf(scattered.x, fmaxf(scattered.y, scattered.z));
+        if (survival <= 0.0f) {
+            terminal = true;
+        } else {
+            if (bounce >= ROULETTE_START) {
+                survival = fminf(1.0f, survival);
+                if (sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_RR)) >= survival) terminal = true;
+                else scattered = scattered / survival;
+            }
+        }
+        if (!terminal) {
+            if (aovOn) {
+                const float record = aov_record_fraction(closure, scatter.lobe);
+                if (record > 0.0f && any_positive(guide)) {
+                    const v3 g = guide * record;
+                    put_pixel(tg.albedo, pixel, g * aovAlbedo, 0.0f, bounce == 0);
+                    put_pixel(tg.normal, pixel, g * shading, 0.0f, bounce == 0);
+                } else if (bounce == 0) {
+                    put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
+                    put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
+                }
+                tg.guide[pixel] = F4((guide * (1.0f - record)) * aov_pass_tint(closure, scatter.lobe), 0.0f);
+            }
+            if (scatter.lobe == LOBE_TRANSMISSION && material.thinWalled == 0) {
+                medium = medium == hit.x ? MEDIUM_NONE : hit.x;
+            }
+            const bool transmitted = scatter.wi.z < 0.0f;
+            p.origin[path] = F4(transmitted ? spawn.below : spawn.position, origin4.w);
+            p.direction[path] = F4(wiWorld, 0.0f);
+            p.throughput[path] = F4(scattered, scatter.pdf);
+            p.state[path] = (scatter.lobe << 24) | (medium & 0xffffffu);
+            push_ray = true;
+        }
+    }
+    if (terminal) {
+        if (aovOn && any_positive(guide)) {
+            put_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f, bounce == 0);
+            put_pixel(tg.normal, pixel, guide * shading, 0.0f, bounce == 0);
+        } else if (aovOn && bounce == 0) {
+            put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
+            put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
+        }
+        add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
+    }
+    return push_ray;
+}
+
 // Launch-shape variants of shade_surface, picked per renderer (the
 // autotuner measures them). The BSDF tables — 67,712 bytes that each
 // closure evaluation reads ~20 times — are staged into LDS once per
@@ -397,7 +614,6 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
         __syncthreads();
         tables = lds_tables;
     }
-    const bool aovOn = tg.albedo != nullptr;
     uint32_t* next_rays = p.ray_queue[(bounce + 1) & 1];
     // Sorted variants: the lane that held each sorted hit before the sort.
     __shared__ uint32_t sorted_home[SORT ? SB : 1];
@@ -433,189 +649,20 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
             if (!valid) path = 0;
         }
         bool push_ray = false;
-        uint32_t pixel = 0;
         if (valid) {
-            pixel = base + path;  // the path's target slot
-            const uint2 key = p.key[path];
-            const uint4 hit = p.hit[path];
-            const float4 dir4 = p.direction[path];
-            const v3 direction = xyz(dir4);
-            const float4 origin4 = p.origin[path];
-            const v3 origin = xyz(origin4);
-            const uint32_t depth_cap = __float_as_uint(origin4.w);
-            const InstanceRecord inst = load_instance(s.instances, hit.x);
-            const Mat material = load_material(s.materials, hit.x);
-            const Spawn spawn = spawn_point(s, inst, hit.y, __uint_as_float(hit.z), __uint_as_float(hit.w), direction);
-            float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
-
-            uint32_t medium = bounce == 0 ? MEDIUM_NONE : (p.state[path] & 0xffffffu);
-            if (medium != MEDIUM_NONE) {
-                const Mat interior = load_material(s.materials, medium);
-                if (interior.transmissionDepth > 0.0f) {
-                    const v3 tc = vclamp(interior.transmissionColor, 1e-4f, 1.0f);
-                    const v3 sigma = (-V3(logf(tc.x), logf(tc.y), logf(tc.z))) / interior.transmissionDepth;
-                    const v3 e = -sigma * length(origin - spawn.position);
-                    thr.x *= expf(e.x);
-                    thr.y *= expf(e.y);
-                    thr.z *= expf(e.z);
-                }
-            }
-            const v3 T = xyz(thr);
-
-            v3 contribution = S3(0.0f);
-            if (inst.light != LIGHT_NONE && spawn.frontFace) {
-                float weight = 1.0f;
-                if (thr.w > 0.0f) {
-                    if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
-                        weight = 0.0f;
-                    } else if (light_sampling == CN_LIGHT_SAMPLING_MIS) {
-                        const v3 delta = spawn.position - origin;
-                        const float cosLight = fabsf(dot(spawn.normal, direction));
-                        const float neePdf = (1.0f - s.env_select_prob) * s.lights[inst.light + hit.y].pdf *
-                                             dot(delta, delta) / cosLight;
-                        weight = power_heuristic(thr.w, neePdf);
-                    }
-                }
-                contribution = (T * material.emission) * weight;
-            }
-            // The emission lands now rather than when the path's fate is
-            // known; a terminal path adds its sample count (and an emission
-            // that was not positive) at the end. A slot sees the same sums.
-            const bool lit = any_positive(contribution);
-            if (bounce == 0) put_pixel(tg.radiance, pixel, lit ? contribution : S3(0.0f), 0.0f, true);
-            else if (lit) add_pixel(tg.radiance, pixel, contribution, 0.0f);
-
-            // The next-event candidate (a point on a light, or an
-            // environment direction) depends only on the hit point, so it is
-            // drawn before the closure is resolved; only the candidate stays
-            // live across the resolution.
-            const float envProb = s.env_select_prob;
-            const bool nee = bounce + 1 < depth_cap && light_sampling != CN_LIGHT_SAMPLING_BSDF &&
-                             (s.light_count > 0 || envProb > 0.0f);
-            LightConnection cand{S3(0.0f), 1e30f, S3(0.0f), 0.0f, false, LIGHT_NONE, 0u};
-            bool cand_env = false;
-            if (nee) {
-                const float uLight = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_LIGHT));
-                const f2 uPoint = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_NEE_POINT));
-                if (uLight < envProb) {
-                    const EnvSample e = env_sample(s, uPoint);
-                    cand.direction = e.direction;
-                    cand.radiance = e.radiance;
-                    cand.pdf = e.pdf;
-                    cand_env = true;
-                } else {
-                    const float uList = (uLight - envProb) / (1.0f - envProb);
-                    cand = sample_lights(s, uList, uPoint, spawn.position);
-                }
-            }
-
-            const v3 shading = spawn.shading;
-            const Frame frame = make_frame(shading);
-            const v3 wo = to_local(frame, -direction);
-            const Closure closure = resolve_closure(tables, material, wo, medium == hit.x);
-
-            v3 guide = S3(0.0f);
-            if (aovOn) {
-                guide = bounce == 0 ? S3(1.0f) : xyz(tg.guide[pixel]);
-                if (bounce == 0) tg.depth[pixel] = 0.0f + length(origin - spawn.position) * dir4.w;
-            }
-            const v3 aovAlbedo = vclamp(closure.aovAlbedo, 0.0f, 1.0f);
-            bool terminal = false;
-
-            if (bounce + 1 >= depth_cap) {
-                terminal = true;
-            } else {
-                // Next-event estimation (shade_surface.slang:363-446), with the
-                // candidate drawn above.
-                bool push_shadow = false;
-                v3 sh_origin = S3(0.0f), sh_contrib = S3(0.0f);
-                if (nee) {
-                    const v3 wiLight = to_local(frame, cand.direction);
-                    const bool through = dot(cand.direction, spawn.normal) < 0.0f;
-                    if (cand.pdf > 0.0f && wiLight.z != 0.0f && (wiLight.z < 0.0f) == through) {
-                        const float neePdf = (cand_env ? envProb : 1.0f - envProb) * cand.pdf;
-                        const BsdfEval light = eval_with_pdf(closure, wiLight, wo);
-                        const float weight = !cand.delta && light_sampling == CN_LIGHT_SAMPLING_MIS
-                                                 ? power_heuristic(neePdf, light.pdf)
-                                                 : 1.0f;
-                        const v3 unshadowed =
-                            (((T * cand.radiance) * light.f) * fabsf(wiLight.z)) * (weight / neePdf);
-                        sh_origin = through ? spawn.below : spawn.position;
-                        if (any_positive(unshadowed)) {
-                            push_shadow = true;
-                            sh_contrib = unshadowed;
-                        }
-                    }
-                }
-                // The connection goes out before scattering, so its record
-                // is dead through the rest of the shading.
-                const uint32_t split = chunk_split(first);
-                const uint32_t ss = split * p.sub_capacity +
-                                    wave_claim(&counters[counter_index(bounce, Q_SHADOW, split)], push_shadow);
-                if (push_shadow) {
-                    p.shadow_origin[ss] = F4(sh_origin, cand.tmax);
-                    p.shadow_dir[ss] = F4(cand.direction, __uint_as_float(pixel));
-                    p.shadow_contrib[ss] = F4(sh_contrib, __uint_as_float(cand.instance));
-                    p.shadow_prim[ss] = cand.primitive;
-                }
-
-                // Scatter (shade_surface.slang:453-489).
-                const float uLobe = sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_LOBE));
-                const f2 u = sample_2d(key.x, key.y, bounce_dim(bounce, SLOT_BSDF_DIR));
-                BsdfSample scatter;
-                BsdfEval drawn{S3(0.0f), 0.0f};
-                if (sample_bsdf_direction(closure, wo, uLobe, u, scatter)) drawn = eval_with_pdf(closure, scatter.wi, wo);
-                scatter.pdf = drawn.pdf;
-                const v3 wiWorld = to_world(frame, scatter.wi);
-                v3 scattered = S3(0.0f);
-                if (scatter.pdf > 0.0f && scatter.wi.z != 0.0f &&
-                    (scatter.wi.z > 0.0f) == (dot(wiWorld, spawn.normal) > 0.0f)) {
-                    scattered = (T * drawn.f) * (fabsf(scatter.wi.z) / scatter.pdf);
-                }
-                float survival = fmaxf(scattered.x, fmaxf(scattered.y, scattered.z));
-                if (survival <= 0.0f) {
-                    terminal = true;
-                } else {
-                    if (bounce >= ROULETTE_START) {
-                        survival = fminf(1.0f, survival);
-                        if (sample_1d(key.x, key.y, bounce_dim(bounce, SLOT_RR)) >= survival) terminal = true;
-                        else scattered = scattered / survival;
-                    }
-                }
-                if (!terminal) {
-                    if (aovOn) {
-                        const float record = aov_record_fraction(closure, scatter.lobe);
-                        if (record > 0.0f && any_positive(guide)) {
-                            const v3 g = guide * record;
-                            put_pixel(tg.albedo, pixel, g * aovAlbedo, 0.0f, bounce == 0);
-                            put_pixel(tg.normal, pixel, g * shading, 0.0f, bounce == 0);
-                        } else if (bounce == 0) {
-                            put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
-                            put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
-                        }
-                        tg.guide[pixel] = F4((guide * (1.0f - record)) * aov_pass_tint(closure, scatter.lobe), 0.0f);
-                    }
-                    if (scatter.lobe == LOBE_TRANSMISSION && material.thinWalled == 0) {
-                        medium = medium == hit.x ? MEDIUM_NONE : hit.x;
-                    }
-                    const bool transmitted = scatter.wi.z < 0.0f;
-                    p.origin[path] = F4(transmitted ? spawn.below : spawn.position, origin4.w);
-                    p.direction[path] = F4(wiWorld, 0.0f);
-                    p.throughput[path] = F4(scattered, scatter.pdf);
-                    p.state[path] = (scatter.lobe << 24) | (medium & 0xffffffu);
-                    push_ray = true;
-                }
-            }
-            if (terminal) {
-                if (aovOn && any_positive(guide)) {
-                    put_pixel(tg.albedo, pixel, guide * aovAlbedo, 0.0f, bounce == 0);
-                    put_pixel(tg.normal, pixel, guide * shading, 0.0f, bounce == 0);
-                } else if (aovOn && bounce == 0) {
-                    put_pixel(tg.albedo, pixel, S3(0.0f), 0.0f, true);
-                    put_pixel(tg.normal, pixel, S3(0.0f), 0.0f, true);
-                }
-                add_pixel(tg.radiance, pixel, lit ? S3(0.0f) : contribution, 1.0f);
-            }
+            push_ray = shade_path(s, p, tg, tables, path, base, bounce, light_sampling,
+                                  [&](bool push, const ShadowRecord& r) {
+                                      const uint32_t split = chunk_split(first);
+                                      const uint32_t ss =
+                                          split * p.sub_capacity +
+                                          wave_claim(&counters[counter_index(bounce, Q_SHADOW, split)], push);
+                                      if (push) {
+                                          p.shadow_origin[ss] = F4(r.origin, r.tmax);
+                                          p.shadow_dir[ss] = F4(r.direction, __uint_as_float(r.pixel));
+                                          p.shadow_contrib[ss] = F4(r.contrib, __uint_as_float(r.instance));
+                                          p.shadow_prim[ss] = r.primitive;
+                                      }
+                                  });
         }
         if constexpr (SORT) {
             // The next bounce's rays go out in the hits' queue order, not in
@@ -630,6 +677,67 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
         const uint32_t split = chunk_split(first);
         const uint32_t rs = wave_claim(&counters[counter_index(bounce + 1, Q_RAY, split)], push_ray);
         if (push_ray) next_rays[split * p.sub_capacity + rs] = path;
+    }
+}
+
+// --- tail: the paths still alive after one bounce's trace, each carried to its end ----
+//
+// Past the first few bounces a wave holds few live paths, and each bounce's
+// three launches last as long as the slowest of them: the deep bounces cost
+// the sum, over bounces, of the slowest path's latency. From a chosen bounce
+// on, one launch takes every path of that bounce's miss and hit queues
+// through all of its remaining bounces — shading, the next-event connection
+// traced at once, the next closest hit — with the wavefront kernels' own
+// device functions. A slot sees the same adds in the same order as in the
+// wavefront (bounce b's shading, then b's connection, then bounce b + 1),
+// so the film is bit for bit the same. Misses come first in item order, so
+// most waves hold one kind of work.
+template <bool W8>
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(TAIL_WAVES))) void k_tail(
+    SceneView s, PathPool p, WaveTargets tg, uint32_t* counters, uint32_t base, uint32_t bounce,
+    uint32_t light_sampling) {
+    __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
+    const uint32_t n_miss = queue_total(counters, bounce, Q_MISS);
+    const uint32_t n = n_miss + queue_total(counters, bounce, Q_HIT);
+    const uint32_t lanes = gridDim.x * TRAVERSAL_BLOCK;
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * TRAVERSAL_BLOCK + threadIdx.x);
+    uint32_t nodes = 0, tris = 0;  // the walks' fetch counts, not kept here
+    for (uint32_t first = blockIdx.x * TRAVERSAL_BLOCK; first < n; first += lanes) {
+        const uint32_t i = first + threadIdx.x;
+        if (i >= n) continue;
+        if (i < n_miss) {
+            const uint32_t path = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
+            miss_path(s, p, tg, path, base, bounce, light_sampling);
+            continue;
+        }
+        const uint32_t path =
+            p.hit_queue[queue_slot(counters, bounce, Q_HIT, i - n_miss, p.sub_capacity)] & HIT_PATH_MASK;
+        for (uint32_t b = bounce;; ++b) {
+            bool connect = false;
+            ShadowRecord sh;
+            const bool next = shade_path(s, p, tg, s.tables, path, base, b, light_sampling,
+                                         [&](bool push, const ShadowRecord& r) {
+                                             connect = push;
+                                             sh = r;
+                                         });
+            if (connect) {
+                // As the trace kernel takes a connection: a solid hit short
+                // of a triangle light occludes without the closest hit.
+                const float t_early = sh.instance == LIGHT_NONE ? 3.0e38f : sh.tmax * 0.998f;
+                const float att = trace_visibility<W8>(s, sh.origin, sh.direction, sh.tmax, sh.instance,
+                                                       sh.primitive, t_early, st, nodes, tris);
+                if (att > 0.0f) add_pixel(tg.radiance, sh.pixel, sh.contrib * att, 0.0f);
+            }
+            if (!next) break;
+            const uint2 key = p.key[path];
+            const TraceHit h = trace_closest<W8>(s, xyz(p.origin[path]), xyz(p.direction[path]), 1e30f, MASK_ALL, st,
+                                                 key.x, key.y, b + 1, nodes, tris);
+            if (!h.hit) {
+                miss_path(s, p, tg, path, base, b + 1, light_sampling);
+                break;
+            }
+            p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+        }
     }
 }
 
@@ -822,7 +930,7 @@ hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t light_sampling, int counting,
+                         uint32_t bounce, uint32_t light_sampling, int counting, bool tail,
                          Profiler* prof) {
     const uint32_t trace_grid = shape.trace_blocks;
     const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
@@ -842,6 +950,20 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     }
 #undef CN_TRACE
     if (prof) prof->end(K_TRACE);
+    if (tail) {
+        // shade_surface's block cap in 256-lane blocks, at most the trace
+        // grid (whose lanes the traversal stack's spill area is sized for).
+        const uint32_t tail_grid = shape.shade_blocks < trace_grid ? shape.shade_blocks : trace_grid;
+        if (prof) prof->begin(K_TAIL);
+        if (wide)
+            hipLaunchKernelGGL(k_tail<true>, dim3(tail_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters,
+                               base, bounce, light_sampling);
+        else
+            hipLaunchKernelGGL(k_tail<false>, dim3(tail_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters,
+                               base, bounce, light_sampling);
+        if (prof) prof->end(K_TAIL);
+        return hipGetLastError();
+    }
     if (prof) prof->begin(K_SHADE_MISS);
     hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
                        light_sampling);

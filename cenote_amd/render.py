@@ -288,7 +288,8 @@ class Renderer:
 
     @property
     def shape(self) -> dict:
-        """The launch shape: trace_blocks, shade_variant, shade_blocks, light_blocks."""
+        """The launch shape: trace_blocks, shade_variant, shade_blocks, light_blocks, bvh_width,
+        tail_bounce."""
         sh = _abi.LaunchShape()
         check(self._lib.cn_renderer_get_shape(self.handle, C.byref(sh)))
         return {name: getattr(sh, name) for name, _ in _abi.LaunchShape._fields_}
@@ -330,6 +331,15 @@ class Renderer:
         tc = _abi.TraversalCounts()
         check(self._lib.cn_renderer_traversal_counts(self.handle, C.byref(tc)))
         return {k: int(getattr(tc, k)) for k, _ in _abi.TraversalCounts._fields_}
+
+    def traversal_counts_by_bounce(self) -> np.ndarray:
+        """(max_bounces, 4) int64: per bounce, ray nodes, ray triangles,
+        shadow nodes, shadow triangles (bounce b's shadow columns count the
+        connections bounce b - 1 made)."""
+        out = np.zeros(4 * self.max_bounces, dtype=np.uint64)
+        check(self._lib.cn_renderer_traversal_counts_by_bounce(
+            self.handle, out.ctypes.data_as(C.POINTER(C.c_uint64)), self.max_bounces))
+        return out.astype(np.int64).reshape(self.max_bounces, 4)
 
     def close(self):
         if self.handle:

@@ -73,6 +73,9 @@ enum {
 #define CN_POOL_CAPACITY (1u << 27)
 /* The largest pool a renderer may ask for. */
 #define CN_POOL_CAPACITY_MAX (1u << 27)
+/* A new renderer's tail bounce (cn_launch_shape.tail_bounce): after bounce
+ * 5's trace one launch finishes every live path. */
+#define CN_DEFAULT_TAIL_BOUNCE 5u
 
 /*
  * One instance's lowered OpenPBR surface, 144 bytes, field for field the
@@ -230,7 +233,7 @@ typedef struct cn_wave_counts {
     uint32_t shadows[256];
 } cn_wave_counts;
 
-/* BVH traversal work of the most recent wave traced with counters on. */
+/* BVH traversal work of the waves traced since counters were last turned on. */
 typedef struct cn_traversal_counts {
     uint64_t ray_nodes;      /* nodes fetched by closest-hit rays */
     uint64_t ray_triangles;  /* triangles tested by closest-hit rays */
@@ -362,6 +365,10 @@ typedef struct cn_launch_shape {
     uint32_t light_blocks;   /* most 256-lane workgroups of shade_miss, 1..65536 */
     uint32_t bvh_width;      /* 4 or 8: which of a scene's trees the traversal walks (a scene
                               * without an 8-wide tree is walked 4-wide) */
+    uint32_t tail_bounce;    /* 0: every bounce as wavefront stages; b in 1..255: after bounce b's
+                              * trace, one launch carries every live path through its remaining
+                              * bounces (the same films, bit for bit; waves with counters on keep
+                              * the stages) */
 } cn_launch_shape;
 int cn_renderer_get_shape(cn_renderer* renderer, cn_launch_shape* out);
 int cn_renderer_set_shape(cn_renderer* renderer, const cn_launch_shape* shape);
@@ -378,6 +385,11 @@ int cn_renderer_graph_stats(cn_renderer* renderer, uint64_t* captures, uint64_t*
 int cn_renderer_set_counters(cn_renderer* renderer, int enabled);
 int cn_renderer_wave_counts(cn_renderer* renderer, cn_wave_counts* out);
 int cn_renderer_traversal_counts(cn_renderer* renderer, cn_traversal_counts* out);
+/* The same counts per bounce: out[4 * b + k] for b < `bounces` (at most
+ * 256), k = ray nodes, ray triangles, shadow nodes, shadow triangles; the
+ * shadow counts of bounce b are those of bounce b - 1's connections, which
+ * bounce b's trace launch walks. */
+int cn_renderer_traversal_counts_by_bounce(cn_renderer* renderer, uint64_t* out, uint32_t bounces);
 /* Time every stage launch with HIP events (measurement only; the
  * reference has no counterpart — it carries no GPU timestamps). */
 int cn_renderer_set_profiling(cn_renderer* renderer, int enabled);

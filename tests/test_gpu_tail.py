@@ -1,0 +1,119 @@
+"""The tail launch changes speed, never results: from any bounce on, the
+kernel that carries every live path through its remaining bounces renders
+bitwise the film of the wavefront stages, AOVs included, for every light
+sampling mode, both BVH widths, translucent and delta-lit scenes, and waves
+that span several pool ranges."""
+
+import numpy as np
+import pytest
+
+import cenote_amd as cn
+from cenote_amd import workloads
+
+from reference_scenes import point_light_scene, translucent_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _sums(ctx, scene, width, height, samples, depth, tail, capacity=0, light_sampling=cn.LightSampling.MIS,
+          bvh_width=4, graphs=True):
+    r = cn.Renderer(ctx, capacity=capacity, max_bounces=depth, light_sampling=light_sampling)
+    r.shape = {"tail_bounce": tail, "bvh_width": bvh_width}
+    r.set_graphs(graphs)
+    film = cn.Film(ctx, width, height)
+    # Two progressive calls, so the second folds into a non-empty film.
+    r.accumulate(scene, film, count=samples)
+    r.accumulate(scene, film, count=samples)
+    sums = film.sums()
+    film.close()
+    r.close()
+    return sums
+
+
+def _assert_same(got, want, label):
+    for k, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g, w, equal_nan=True), (label, k, int(np.sum(g != w)))
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    made = {name: cn.Scene(ctx, workloads.build(name)) for name in ("cornell", "chart", "heavy")}
+    made["translucent"] = cn.Scene(ctx, translucent_scene())
+    made["point"] = cn.Scene(ctx, point_light_scene())
+    yield made
+    for s in made.values():
+        s.close()
+
+
+@pytest.mark.parametrize("name,width,height,samples,depth,tails", [
+    ("cornell", 64, 64, 4, 8, (1, 2, 3, 7, 8)),
+    ("chart", 128, 72, 3, 12, (1, 3, 5, 11)),
+    ("heavy", 160, 90, 2, 16, (1, 4, 6, 15)),
+    ("translucent", 48, 48, 4, 8, (1, 2, 5)),
+    ("point", 48, 48, 4, 6, (1, 3)),
+])
+def test_tail_renders_the_wavefront_film(ctx, scenes, name, width, height, samples, depth, tails):
+    print(f"tail: {name} {width}x{height} spp {samples} depth {depth}", flush=True)
+    want = _sums(ctx, scenes[name], width, height, samples, depth, 0)
+    for tail in tails:
+        print(f"tail: {name} from bounce {tail}", flush=True)
+        _assert_same(_sums(ctx, scenes[name], width, height, samples, depth, tail), want, (name, tail))
+
+
+@pytest.mark.parametrize("mode", [cn.LightSampling.BSDF_ONLY, cn.LightSampling.NEE_ONLY])
+def test_tail_keeps_every_light_sampling_mode(ctx, scenes, mode):
+    print(f"tail: chart light sampling {mode}", flush=True)
+    want = _sums(ctx, scenes["chart"], 96, 64, 2, 12, 0, light_sampling=mode)
+    _assert_same(_sums(ctx, scenes["chart"], 96, 64, 2, 12, 2, light_sampling=mode), want, mode)
+
+
+def test_tail_on_the_eight_wide_tree_and_without_graphs(ctx, scenes):
+    print("tail: heavy, 8-wide tree, graphs off", flush=True)
+    want = _sums(ctx, scenes["heavy"], 128, 72, 2, 16, 0, bvh_width=8)
+    _assert_same(_sums(ctx, scenes["heavy"], 128, 72, 2, 16, 3, bvh_width=8, graphs=False), want, "w8")
+
+
+def test_tail_over_several_pool_ranges(ctx, scenes):
+    # A pool of a third of the wave's paths: three ranges, each with its own tail launch.
+    print("tail: chart over three pool ranges", flush=True)
+    w, h, n = 96, 64, 3
+    want = _sums(ctx, scenes["chart"], w, h, n, 12, 0, capacity=w * h)
+    _assert_same(_sums(ctx, scenes["chart"], w, h, n, 12, 4, capacity=w * h), want, "ranges")
+
+
+def test_tail_is_timed_and_counting_keeps_every_bounce(ctx, scenes):
+    print("tail: profiling and counters", flush=True)
+    r = cn.Renderer(ctx, max_bounces=12)
+    r.shape = {"tail_bounce": 3}
+    film = cn.Film(ctx, 96, 64)
+    r.set_profiling(True)
+    r.accumulate(scenes["chart"], film, count=2)
+    times = r.kernel_times()
+    r.set_profiling(False)
+    assert times["tail"][1] == 1 and times["trace"][1] == 4
+    assert times["shade_surface"][1] == 3 and times["shade_miss"][1] == 3
+    # Counting waves run every bounce as stages, so the queue audit is whole.
+    r.set_counters(True)
+    film.reset()
+    r.accumulate(scenes["chart"], film, count=1)
+    counted = r.wave_counts()
+    r.set_counters(False)
+    plain = cn.Renderer(ctx, max_bounces=12)
+    plain.set_counters(True)
+    film.reset()
+    plain.accumulate(scenes["chart"], film, count=1)
+    reference = plain.wave_counts()
+    plain.close()
+    for key in counted:
+        assert np.array_equal(counted[key], reference[key]), key
+    film.close()
+    r.close()
+
+
+def test_invalid_tail_bounce_is_rejected(ctx):
+    r = cn.Renderer(ctx, max_bounces=4)
+    before = r.shape
+    with pytest.raises(cn.CenoteError):
+        r.shape = {"tail_bounce": 256}
+    assert r.shape == before
+    r.close()
