@@ -938,7 +938,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_collapse_write4(const BinaryNod
         BvhNode4Q q{};
         const float* los[3] = {lx, ly, lz};
         const float* his[3] = {hx, hy, hz};
-        float origin[3];
+        float origin[3], steps[3];
         uint32_t planes_lo[3] = {0, 0, 0}, planes_hi[3] = {0, 0, 0};
         for (int a = 0; a < 3; ++a) {
             uint32_t ql[4] = {0, 0, 0, 0}, qh[4] = {0, 0, 0, 0}, eb;
@@ -947,18 +947,20 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_collapse_write4(const BinaryNod
                 planes_lo[a] |= ql[c] << (8 * c);
                 planes_hi[a] |= qh[c] << (8 * c);
             }
-            q.exps |= eb << (8 * a);
+            steps[a] = __uint_as_float(eb << 23);  // the biased exponent's power of two
         }
         q.ox = origin[0];
         q.oy = origin[1];
         q.oz = origin[2];
+        q.step_x = steps[0];
+        q.step_y = steps[1];
+        q.step_z = steps[2];
         q.lo_x = planes_lo[0];
         q.hi_x = planes_hi[0];
         q.lo_y = planes_lo[1];
         q.hi_y = planes_hi[1];
         q.lo_z = planes_lo[2];
         q.hi_z = planes_hi[2];
-        q.depth = w.depth;
         q.child = n.child;
         quant[w.index] = q;
     }

@@ -359,7 +359,7 @@ static BvhNode4Q quantize_node(const BvhNode4& n) {
     const float* hi[3] = {&n.hi_x.x, &n.hi_y.x, &n.hi_z.x};
     const uint32_t* child = &n.child.x;
     BvhNode4Q q{};
-    float origin[3];
+    float origin[3], steps[3];
     uint32_t planes_lo[3] = {0, 0, 0}, planes_hi[3] = {0, 0, 0};
     for (int k = 0; k < 3; ++k) {
         double a = INFINITY, b = -INFINITY;
@@ -391,18 +391,20 @@ static BvhNode4Q quantize_node(const BvhNode4& n) {
             planes_lo[k] |= std::min(ql, 255u) << (8 * c);
             planes_hi[k] |= std::min(qh, 255u) << (8 * c);
         }
-        q.exps |= uint32_t(e + 127) << (8 * k);
+        steps[k] = float(step);  // 2^e, -126 <= e <= 127: exact
     }
     q.ox = origin[0];
     q.oy = origin[1];
     q.oz = origin[2];
+    q.step_x = steps[0];
+    q.step_y = steps[1];
+    q.step_z = steps[2];
     q.lo_x = planes_lo[0];
     q.hi_x = planes_hi[0];
     q.lo_y = planes_lo[1];
     q.hi_y = planes_hi[1];
     q.lo_z = planes_lo[2];
     q.hi_z = planes_hi[2];
-    q.depth = n.meta.x;
     q.child = n.child;
     return q;
 }

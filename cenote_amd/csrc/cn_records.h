@@ -86,15 +86,16 @@ static_assert(sizeof(BvhNode4) == 128, "one 128-byte line per node");
 // The node the traversal reads: the same four children with their bounds
 // quantized to 8 bits per plane on a per-node grid, 64 bytes (four dwordx4
 // fetches, half the full-precision node). Child c's box on axis k spans
-//   origin_k + q_lo * 2^(e_k - 127)  ..  origin_k + q_hi * 2^(e_k - 127)
-// with q the byte c of the plane's word and e_k byte k of `exps`; the host
-// rounds every plane outward (floor / ceil in exact arithmetic), so each
-// quantized box contains the full-precision one.
+//   origin_k + q_lo * step_k  ..  origin_k + q_hi * step_k
+// with q the byte c of the plane's word and step_k a power of two (a normal
+// float, stored as one so the traversal scales by it with one multiply);
+// the host rounds every plane outward (floor / ceil in exact arithmetic),
+// so each quantized box contains the full-precision one.
 struct alignas(16) BvhNode4Q {
-    float ox, oy, oz;
-    uint32_t exps;                    // bytes 0..2: biased exponents of the x, y, z grid steps
+    float ox, oy, oz, step_x;
     uint32_t lo_x, hi_x, lo_y, hi_y;  // byte c: child c's plane, in grid steps from the origin
-    uint32_t lo_z, hi_z, depth, pad;
+    uint32_t lo_z, hi_z;
+    float step_y, step_z;
     uint4 child;
 };
 static_assert(sizeof(BvhNode4Q) == 64, "half a 128-byte line per node");

@@ -27,36 +27,51 @@ constexpr uint32_t NODE_DONE = CHILD_EMPTY;  // never pushed: empty slots fail e
 // global memory (address space 1), so each access is a ds_* or global_*
 // instruction; through plain pointers the compiler merged the two loads of
 // a pop into one flat load, which waits on every outstanding global load
-// too.
+// too. The top is kept as the LDS byte offset of the lane's next free entry
+// (entry count * STACK_ROW + threadIdx.x * 4), so an LDS access needs no
+// address arithmetic of its own.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) char lds_char;
 typedef __attribute__((address_space(1))) uint32_t global_u32;
+
+constexpr uint32_t STACK_ROW = TRAVERSAL_BLOCK * 4;  // bytes between a lane's consecutive LDS entries
+constexpr uint32_t STACK_LDS_END = STACK_LDS * STACK_ROW;
 
 struct Stack {
     lds_u32* lds;
     global_u32* spill;
     uint32_t lanes;
     uint32_t lane;
-    uint32_t sp;
+    uint32_t base;  // this lane's entry 0: threadIdx.x * 4 (the stack is empty when at == base)
+    uint32_t at;    // byte offset of the next free entry
+    __device__ __forceinline__ lds_u32& slot(uint32_t offset) {
+        return *reinterpret_cast<lds_u32*>(reinterpret_cast<lds_char*>(lds) + offset);
+    }
+    __device__ __forceinline__ global_u32& spilled(uint32_t offset) {
+        return spill[((offset - base) / STACK_ROW - STACK_LDS) * lanes + lane];
+    }
+    __device__ __forceinline__ void clear() { at = base; }
     __device__ __forceinline__ void push(uint32_t v) {
-        if (sp >= STACK_MAX) return;
-        if (sp < STACK_LDS) lds[sp * TRAVERSAL_BLOCK + threadIdx.x] = v;
-        else spill[(sp - STACK_LDS) * lanes + lane] = v;
-        ++sp;
+        if (at - base >= STACK_MAX * STACK_ROW) return;
+        if (at < STACK_LDS_END) slot(at) = v;
+        else spilled(at) = v;
+        at += STACK_ROW;
     }
     // Pushes the hit children after the nearest, farthest first: c3, c2,
-    // c1 for nh = 4, c2, c1 for 3, c1 for 2. Away from the LDS boundary the
-    // three stores go out unconditionally; a store past the new top lands
-    // above it, where nothing reads.
+    // c1 for nh = 4, c2, c1 for 3, c1 for 2 (nh >= 1). Away from the LDS
+    // boundary the three stores go out unconditionally, c3 at the top, c2
+    // and c1 at the top plus (nh - 3) and (nh - 2) entries clamped at 0:
+    // a child past the hit count lands on the top before a hit child
+    // overwrites it, or above the new top, where nothing reads.
     __device__ __forceinline__ void push_sorted(uint32_t c3, uint32_t c2, uint32_t c1, uint32_t nh) {
-        if (sp + 3 <= STACK_LDS) {
-            uint32_t at = sp;
-            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c3;
-            at += nh > 3 ? 1u : 0u;
-            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c2;
-            at += nh > 2 ? 1u : 0u;
-            lds[at * TRAVERSAL_BLOCK + threadIdx.x] = c1;
-            at += nh > 1 ? 1u : 0u;
-            sp = at;
+        if (at < (STACK_LDS - 2) * STACK_ROW) {  // entries top .. top + 2 are in LDS
+            const uint32_t n = nh * STACK_ROW;
+            const uint32_t up2 = n > 3 * STACK_ROW ? n - 3 * STACK_ROW : 0u;
+            const uint32_t up1 = n > 2 * STACK_ROW ? n - 2 * STACK_ROW : 0u;
+            slot(at) = c3;
+            slot(at + up2) = c2;
+            slot(at + up1) = c1;
+            at += n - STACK_ROW;
             return;
         }
         if (nh > 3) push(c3);
@@ -66,14 +81,14 @@ struct Stack {
     // The 8-wide form: ref[1..7] pushed farthest first, the ones past the
     // hit count landing above the new top.
     __device__ __forceinline__ void push_sorted8(const uint32_t (&ref)[8], uint32_t nh) {
-        if (sp + 7 <= STACK_LDS) {
-            uint32_t at = sp;
+        if (at < (STACK_LDS - 6) * STACK_ROW) {  // entries top .. top + 6 are in LDS
+            uint32_t a = at;
 #pragma unroll
             for (int k = 7; k >= 1; --k) {
-                lds[at * TRAVERSAL_BLOCK + threadIdx.x] = ref[k];
-                at += uint32_t(k) < nh ? 1u : 0u;
+                slot(a) = ref[k];
+                a += uint32_t(k) < nh ? STACK_ROW : 0u;
             }
-            sp = at;
+            at = a;
             return;
         }
 #pragma unroll
@@ -81,14 +96,15 @@ struct Stack {
             if (uint32_t(k) < nh) push(ref[k]);
     }
     __device__ __forceinline__ uint32_t pop_or_done() {
-        if (sp == 0) return NODE_DONE;
-        --sp;
-        return sp < STACK_LDS ? lds[sp * TRAVERSAL_BLOCK + threadIdx.x] : spill[(sp - STACK_LDS) * lanes + lane];
+        if (at == base) return NODE_DONE;
+        at -= STACK_ROW;
+        return at < STACK_LDS_END ? uint32_t(slot(at)) : uint32_t(spilled(at));
     }
 };
 
 __device__ __forceinline__ Stack make_stack(uint32_t* lds_stack, uint32_t* spill, uint32_t lanes, uint32_t lane) {
-    return Stack{(lds_u32*)lds_stack, (global_u32*)spill, lanes, lane, 0};
+    const uint32_t base = threadIdx.x * 4;
+    return Stack{(lds_u32*)lds_stack, (global_u32*)spill, lanes, lane, base, base};
 }
 
 __device__ __forceinline__ float safe_rcp(float x) {
@@ -216,9 +232,9 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 __device__ __forceinline__ uint32_t visit(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                           float bound, Stack& st) {
     const BvhNode4Q n = nodes[index];
-    const float sx = __uint_as_float((n.exps & 0xffu) << 23) * r.ix;
-    const float sy = __uint_as_float(((n.exps >> 8) & 0xffu) << 23) * r.iy;
-    const float sz = __uint_as_float(((n.exps >> 16) & 0xffu) << 23) * r.iz;
+    const float sx = n.step_x * r.ix;
+    const float sy = n.step_y * r.iy;
+    const float sz = n.step_z * r.iz;
     const float bx0 = (n.ox - r.x) * r.ix, by0 = (n.oy - r.y) * r.iy, bz0 = (n.oz - r.z) * r.iz;
     const uint32_t near_x = r.nx ? n.hi_x : n.lo_x, far_x = r.nx ? n.lo_x : n.hi_x;
     const uint32_t near_y = r.ny ? n.hi_y : n.lo_y, far_y = r.ny ? n.lo_y : n.hi_y;
@@ -314,7 +330,7 @@ __device__ __forceinline__ auto tree_of(const SceneView& s) {
 template <typename Node, typename Leaf>
 __device__ __forceinline__ void walk(const Node* __restrict__ nodes, const SlabRay& r, float& bound, Stack& st,
                                      uint32_t& nodes_visited, Leaf&& leaf_fn) {
-    st.sp = 0;
+    st.clear();
     uint32_t cur = 0;  // the root is always an internal node
     uint32_t leaf = NODE_DONE;
     while (true) {

@@ -31,9 +31,14 @@ def quantize(rec):
 
 
 def decode(q):
+    """Boxes, grid steps, the steps' biased exponents and child refs of
+    64-byte nodes: origin and x step in words 0..3, the byte planes in
+    words 4..9, the y and z steps in words 10..11, the refs in 12..15."""
     origin = q[:, 0:3].view(np.float32).astype(np.float64)
-    exps = np.stack([(q[:, 3] >> (8 * k)) & 0xFF for k in range(3)], axis=1).astype(np.int64)
-    step = np.ldexp(1.0, exps - 127)
+    step = q[:, [3, 10, 11]].view(np.float32).astype(np.float64)
+    mantissa, exponent = np.frexp(step)
+    assert np.all(mantissa == 0.5), "every grid step is a power of two"
+    exps = (exponent - 1 + 127).astype(np.int64)
     planes = q[:, [4, 5, 6, 7, 8, 9]]  # lo_x hi_x lo_y hi_y lo_z hi_z
     byte = np.stack([(planes >> (8 * c)) & 0xFF for c in range(4)], axis=2).astype(np.float64)  # (N, 6, 4)
     lo = np.stack([origin[:, k, None] + byte[:, 2 * k] * step[:, k, None] for k in range(3)], axis=2)

@@ -40,7 +40,7 @@ def test_quantized_boxes_contain_the_full_precision_boxes(scale, offset):
     extent = big - small
     assert np.all(255.0 * step >= extent)
     assert np.all((127.5 * step < extent) | (exps == 1))
-    assert np.array_equal(q[:, 10], np.arange(len(q), dtype=np.uint32) % 40)
+    assert np.all(step >= 2.0 ** -126)  # normal floats: the traversal scales by them exactly
 
 
 def test_flat_and_point_children_stay_inside_their_grid():
