@@ -184,91 +184,6 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     p.key[i] = make_uint2(pixel, sample);
 }
 
-// --- trace: intersect (intersect.slang:45-88) fused with the previous
-// bounce's trace_shadow (trace_shadow.slang:27-72) ------------------------------
-//
-// Bounce b's extension rays and bounce b-1's shadow rays are independent, so
-// one launch walks both: items [0, n_ray) are rays, [n_ray, n_ray + n_shadow)
-// are shadow records. The reference's pass order is kept where it shows:
-// a slot's shadow add from bounce b-1 lands before bounce b's shading adds.
-// Two half-empty launches per bounce become one fuller one, which is what
-// the deep, sparse bounces need.
-
-// The 8-wide walk holds eight keys and refs through its sorting network:
-// at eight waves per SIMD (64 VGPRs) it spills, so it runs at fewer.
-#ifndef TRACE8_WAVES
-#define TRACE8_WAVES 7
-#endif
-// The tail kernel holds shading's registers and a traversal: three waves
-// per SIMD, as shade_surface.
-#ifndef TAIL_WAVES
-#define TAIL_WAVES 3
-#endif
-
-// COUNT: accumulate node and triangle fetches (the counting pass); the
-// timed form keeps no counters. Eight waves per SIMD (64 VGPRs; LDS 8 x 16
-// KB per CU) fit without scratch: traversal waits on memory, so every
-// extra wave hides latency.
-template <bool COUNT, bool W8>
-__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(W8 ? TRACE8_WAVES : 8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
-                                                           uint32_t range_count, uint32_t bounce) {
-    __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
-    // Bounce 0 traces the camera paths: item i is path i.
-    const uint32_t n_ray = bounce == 0 ? range_count : queue_total(counters, bounce, Q_RAY);
-    const uint32_t n = n_ray + (bounce == 0 ? 0u : queue_total(counters, bounce - 1, Q_SHADOW));
-    const uint32_t* rays = p.ray_queue[bounce & 1];
-    const uint32_t lanes = gridDim.x * TRAVERSAL_BLOCK;
-    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * TRAVERSAL_BLOCK + threadIdx.x);
-    const uint32_t mask = bounce == 0 ? MASK_CAMERA : MASK_ALL;
-    uint32_t ray_nodes = 0, ray_tris = 0, shadow_nodes = 0, shadow_tris = 0;
-    for (uint32_t first = blockIdx.x * TRAVERSAL_BLOCK; first < n; first += lanes) {
-        const uint32_t i = first + threadIdx.x;
-        const bool is_ray = i < n_ray;
-        uint32_t path = 0, cls = 0;
-        bool hit = false;
-        if (is_ray) {
-            path = bounce == 0 ? i : rays[queue_slot(counters, bounce, Q_RAY, i, p.sub_capacity)];
-            const v3 o = xyz(p.origin[path]);
-            const v3 d = xyz(p.direction[path]);
-            const uint2 key = p.key[path];
-            const TraceHit h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
-            hit = h.hit;
-            if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
-            cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
-        } else if (i < n) {
-            const uint32_t j = queue_slot(counters, bounce - 1, Q_SHADOW, i - n_ray, p.sub_capacity);
-            const float4 o4 = p.shadow_origin[j];
-            const float4 d4 = p.shadow_dir[j];
-            const float4 c4 = p.shadow_contrib[j];
-            const uint32_t target = __float_as_uint(c4.w);
-            // Any solid non-target hit short of the light's own distance
-            // (tmax / 1.001 for a triangle target) lies in front of the
-            // target, so it occludes without finding the closest hit.
-            const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
-            const float att = trace_visibility<W8>(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[j], t_early, st,
-                                               shadow_nodes, shadow_tris);
-            if (att > 0.0f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
-        }
-        if (first < n_ray) {  // uniform per block: some wave of this block holds rays
-            const uint32_t split = chunk_split(first);
-            const uint32_t hs = wave_claim(&counters[counter_index(bounce, Q_HIT, split)], is_ray && hit);
-            const uint32_t ms = wave_claim(&counters[counter_index(bounce, Q_MISS, split)], is_ray && !hit);
-            if (is_ray) {
-                const uint32_t at = split * p.sub_capacity;
-                if (hit) p.hit_queue[at + hs] = path | (cls << HIT_PATH_BITS);
-                else p.miss_queue[at + ms] = path;
-            }
-        }
-    }
-    if constexpr (COUNT) {
-        unsigned long long* c = p.traversal_counts + 4 * bounce;  // this launch's bounce row
-        wave_add_u64(&c[0], ray_nodes);
-        wave_add_u64(&c[1], ray_tris);
-        wave_add_u64(&c[2], shadow_nodes);
-        wave_add_u64(&c[3], shadow_tris);
-    }
-}
-
 // --- shade_miss (shade_miss.slang:27-75) ----------------------------------------------
 
 // One path whose ray at `bounce` escaped: the environment's radiance, MIS
@@ -315,6 +230,100 @@ __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, W
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         const uint32_t path = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
         miss_path(s, p, tg, path, base, bounce, light_sampling);
+    }
+}
+
+// --- trace: intersect (intersect.slang:45-88) fused with the previous
+// bounce's trace_shadow (trace_shadow.slang:27-72) ------------------------------
+//
+// Bounce b's extension rays and bounce b-1's shadow rays are independent, so
+// one launch walks both: items [0, n_ray) are rays, [n_ray, n_ray + n_shadow)
+// are shadow records. The reference's pass order is kept where it shows:
+// a slot's shadow add from bounce b-1 lands before bounce b's shading adds.
+// Two half-empty launches per bounce become one fuller one, which is what
+// the deep, sparse bounces need.
+
+// The 8-wide walk holds eight keys and refs through its sorting network:
+// at eight waves per SIMD (64 VGPRs) it spills, so it runs at fewer.
+#ifndef TRACE8_WAVES
+#define TRACE8_WAVES 7
+#endif
+// The tail kernel holds shading's registers and a traversal: three waves
+// per SIMD, as shade_surface.
+#ifndef TAIL_WAVES
+#define TAIL_WAVES 3
+#endif
+
+// COUNT: accumulate node and triangle fetches (the counting pass); the
+// timed form keeps no counters. Eight waves per SIMD (64 VGPRs; LDS 8 x 16
+// KB per CU) fit without scratch: traversal waits on memory, so every
+// extra wave hides latency.
+// CAMERA: the bounce-0 form, which also shades the camera rays that escape
+// (a separate instantiation, so the later bounces' kernel keeps its
+// registers).
+template <bool COUNT, bool W8, bool CAMERA>
+__global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(W8 ? TRACE8_WAVES : 8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                           uint32_t base, uint32_t range_count, uint32_t bounce,
+                                                           uint32_t light_sampling) {
+    __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
+    // Bounce 0 traces the camera paths: item i is path i.
+    const uint32_t n_ray = CAMERA ? range_count : queue_total(counters, bounce, Q_RAY);
+    const uint32_t n = n_ray + (CAMERA ? 0u : queue_total(counters, bounce - 1, Q_SHADOW));
+    const uint32_t* rays = p.ray_queue[bounce & 1];
+    const uint32_t lanes = gridDim.x * TRAVERSAL_BLOCK;
+    Stack st = make_stack(lds_stack, p.stack_overflow, lanes, blockIdx.x * TRAVERSAL_BLOCK + threadIdx.x);
+    const uint32_t mask = CAMERA ? MASK_CAMERA : MASK_ALL;
+    uint32_t ray_nodes = 0, ray_tris = 0, shadow_nodes = 0, shadow_tris = 0;
+    for (uint32_t first = blockIdx.x * TRAVERSAL_BLOCK; first < n; first += lanes) {
+        const uint32_t i = first + threadIdx.x;
+        const bool is_ray = i < n_ray;
+        uint32_t path = 0, cls = 0;
+        bool hit = false;
+        if (is_ray) {
+            path = CAMERA ? i : rays[queue_slot(counters, bounce, Q_RAY, i, p.sub_capacity)];
+            const v3 o = xyz(p.origin[path]);
+            const v3 d = xyz(p.direction[path]);
+            const uint2 key = p.key[path];
+            const TraceHit h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
+            hit = h.hit;
+            if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+            cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
+        } else if (i < n) {
+            const uint32_t j = queue_slot(counters, bounce - 1, Q_SHADOW, i - n_ray, p.sub_capacity);
+            const float4 o4 = p.shadow_origin[j];
+            const float4 d4 = p.shadow_dir[j];
+            const float4 c4 = p.shadow_contrib[j];
+            const uint32_t target = __float_as_uint(c4.w);
+            // Any solid non-target hit short of the light's own distance
+            // (tmax / 1.001 for a triangle target) lies in front of the
+            // target, so it occludes without finding the closest hit.
+            const float t_early = target == LIGHT_NONE ? 3.0e38f : o4.w * 0.998f;
+            const float att = trace_visibility<W8>(s, xyz(o4), xyz(d4), o4.w, target, p.shadow_prim[j], t_early, st,
+                                               shadow_nodes, shadow_tris);
+            if (att > 0.0f) add_pixel(tg.radiance, __float_as_uint(d4.w), xyz(c4) * att, 0.0f);
+        }
+        if (first < n_ray) {  // uniform per block: some wave of this block holds rays
+            const uint32_t split = chunk_split(first);
+            const uint32_t hs = wave_claim(&counters[counter_index(bounce, Q_HIT, split)], is_ray && hit);
+            const uint32_t ms = wave_claim(&counters[counter_index(bounce, Q_MISS, split)], is_ray && !hit);
+            if (is_ray) {
+                const uint32_t at = split * p.sub_capacity;
+                if (hit) p.hit_queue[at + hs] = path | (cls << HIT_PATH_BITS);
+                else if (!CAMERA) p.miss_queue[at + ms] = path;  // bounce 0's are counted, and shaded below
+            }
+        }
+        // A camera ray that escaped is shaded here: no other item of this
+        // launch adds to its slot (bounce 0 carries no connections).
+        if constexpr (CAMERA) {
+            if (is_ray && !hit) miss_path(s, p, tg, path, base, 0, light_sampling);
+        }
+    }
+    if constexpr (COUNT) {
+        unsigned long long* c = p.traversal_counts + 4 * bounce;  // this launch's bounce row
+        wave_add_u64(&c[0], ray_nodes);
+        wave_add_u64(&c[1], ray_tris);
+        wave_add_u64(&c[2], shadow_nodes);
+        wave_add_u64(&c[3], shadow_tris);
     }
 }
 
@@ -938,16 +947,20 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     // last bounce shades terminally and leaves no shadow rays behind.
     if (prof) prof->begin(K_TRACE);
     const bool wide = shape.bvh_width == 8 && s.nodes8 != nullptr;
-#define CN_TRACE(COUNT, W8)                                                                                          \
-    hipLaunchKernelGGL((k_trace<COUNT, W8>), dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters, \
-                       range_count, bounce)
+#define CN_TRACE(COUNT, W8, CAMERA)                                                                                 \
+    hipLaunchKernelGGL((k_trace<COUNT, W8, CAMERA>), dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg,   \
+                       counters, base, range_count, bounce, light_sampling)
+#define CN_TRACE_CAMERA(COUNT, W8)   \
+    if (bounce == 0) CN_TRACE(COUNT, W8, true); \
+    else CN_TRACE(COUNT, W8, false)
     if (counting) {
-        if (wide) CN_TRACE(true, true);
-        else CN_TRACE(true, false);
+        if (wide) CN_TRACE_CAMERA(true, true);
+        else CN_TRACE_CAMERA(true, false);
     } else {
-        if (wide) CN_TRACE(false, true);
-        else CN_TRACE(false, false);
+        if (wide) CN_TRACE_CAMERA(false, true);
+        else CN_TRACE_CAMERA(false, false);
     }
+#undef CN_TRACE_CAMERA
 #undef CN_TRACE
     if (prof) prof->end(K_TRACE);
     if (tail) {
@@ -964,10 +977,12 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
         if (prof) prof->end(K_TAIL);
         return hipGetLastError();
     }
-    if (prof) prof->begin(K_SHADE_MISS);
-    hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
-                       light_sampling);
-    if (prof) prof->end(K_SHADE_MISS);
+    if (bounce > 0) {  // bounce 0's misses were shaded by the trace launch
+        if (prof) prof->begin(K_SHADE_MISS);
+        hipLaunchKernelGGL(k_shade_miss, dim3(light_grid), dim3(BLOCK), 0, stream, s, p, tg, counters, base, bounce,
+                           light_sampling);
+        if (prof) prof->end(K_SHADE_MISS);
+    }
     if (prof) prof->begin(K_SHADE_SURFACE);
     const uint32_t sb = shade_block(shape.shade_variant);
     uint32_t shade_grid = (range_count + sb - 1) / sb;

@@ -91,7 +91,8 @@ def test_tail_is_timed_and_counting_keeps_every_bounce(ctx, scenes):
     times = r.kernel_times()
     r.set_profiling(False)
     assert times["tail"][1] == 1 and times["trace"][1] == 4
-    assert times["shade_surface"][1] == 3 and times["shade_miss"][1] == 3
+    # Bounce 0's misses are shaded by its trace launch.
+    assert times["shade_surface"][1] == 3 and times["shade_miss"][1] == 2
     # Counting waves run every bounce as stages, so the queue audit is whole.
     r.set_counters(True)
     film.reset()
