@@ -211,6 +211,38 @@ __host__ __device__ inline uint32_t band_index(uint32_t pixel, uint32_t width, u
     return band * (width * 8u) + x * rows + (y - band * 8u);
 }
 
+// A job's slot layout: its pixels (in band order) go in groups of
+// SLOT_GROUP consecutive indices, and a group's slots hold its samples one
+// after another, sample k of the group's pixel j at k * g + j (g the
+// group's pixel count: SLOT_GROUP, fewer only in a job's last group). With
+// groups of one pixel each pixel's samples are consecutive, so a wave64 of
+// camera paths holds one to a few neighbouring pixels' samples, whose rays
+// walk nearly the same nodes; a 1080p frame traced this way took 15.2 ms
+// against 16.0 with groups of 8 pixels and 16.9 with the samples of a view
+// one after another (every pixel of sample 0, then of sample 1, ...). With
+// one sample the layout is band order itself.
+#ifndef SLOT_GROUP
+#define SLOT_GROUP 1u
+#endif
+__host__ __device__ inline uint32_t job_slot(uint32_t index, uint32_t k, uint32_t pixels, uint32_t samples) {
+    const uint32_t first = index - index % SLOT_GROUP;
+    const uint32_t left = pixels - first;
+    const uint32_t g = left < SLOT_GROUP ? left : SLOT_GROUP;
+    return first * samples + k * g + (index - first);
+}
+// The inverse: slot `local` of a job -> (band-order index, sample).
+__host__ __device__ inline void job_unslot(uint32_t local, uint32_t pixels, uint32_t samples, uint32_t* index,
+                                           uint32_t* k) {
+    const uint32_t full = pixels / SLOT_GROUP;  // groups of SLOT_GROUP pixels, then one shorter group
+    const uint32_t span = SLOT_GROUP * samples;
+    const uint32_t group = local / span;
+    const uint32_t first = (group < full ? group : full) * SLOT_GROUP;
+    const uint32_t g = group < full ? SLOT_GROUP : pixels - first;
+    const uint32_t r = local - first * samples;
+    *k = r / g;
+    *index = first + (r - *k * g);
+}
+
 // The SoA path pool (shaders/pathstate.slang:207-232). A wave is a set of
 // jobs — (view, consecutive samples) pairs — laid end to end in a slot
 // space; a path's target slot is its range base plus its pool index, and
@@ -258,9 +290,10 @@ struct CameraView {
 
 // One job of a wave: `samples` consecutive samples, starting at
 // `first_sample`, of every pixel of one view, occupying slots
-// [slot_base, slot_base + samples * pixels) sample-major. When `beauty_sum`
-// is set, the accumulate kernel folds the job's slots into that film's sums
-// in sample order (render/film.rs:66-127, accumulate.slang:46-67).
+// [slot_base, slot_base + samples * pixels) as laid out by job_slot. When
+// `beauty_sum` is set, the accumulate kernel folds the job's slots into
+// that film's sums in sample order (render/film.rs:66-127,
+// accumulate.slang:46-67).
 constexpr uint32_t JOB_RESET = 1u;  // the job's first sample overwrites the sums
 
 struct JobRecord {

@@ -170,9 +170,9 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
         else hi = mid;
     }
     const JobRecord& job = jobs[lo];
-    const uint32_t local = slot - job.slot_base;
-    const uint32_t k = local / job.pixels;
-    const uint32_t pixel = band_pixel(local - k * job.pixels, job.width, job.height);
+    uint32_t index, k;
+    job_unslot(slot - job.slot_base, job.pixels, job.samples, &index, &k);
+    const uint32_t pixel = band_pixel(index, job.width, job.height);
     const uint32_t sample = job.first_sample + k;
     v3 o, d;
     float cf;
@@ -760,60 +760,174 @@ __device__ __forceinline__ float4 guarded(float4 c) {
 
 __device__ __forceinline__ float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 
+// The film fold of one pixel: its running sums, updated sample by sample
+// in sample order. Each sum sees only its own target's samples, so the
+// targets may be folded one after another.
+struct PixelSums {
+    float4 b, a, nn;
+    float d, m;
+    bool reset, aov, moments;
+    __device__ __forceinline__ void radiance(uint32_t k, float4 cb) {
+        const bool first = reset && k == 0;
+        cb = guarded(cb);
+        b = first ? cb : add4(b, cb);
+        if (moments) {
+            const float y = (cb.x * LUMA_R + cb.y * LUMA_G) + cb.z * LUMA_B;
+            m = first ? y * y : m + y * y;
+        }
+    }
+    __device__ __forceinline__ void albedo(uint32_t k, float4 ca) {
+        ca = guarded(ca);
+        a = (reset && k == 0) ? ca : add4(a, ca);
+    }
+    __device__ __forceinline__ void normal(uint32_t k, float4 cn) {
+        cn = guarded(cn);
+        nn = (reset && k == 0) ? cn : add4(nn, cn);
+    }
+    __device__ __forceinline__ void depth(uint32_t k, float cd) {
+        if (isnan(cd)) cd = 0.0f;
+        d = (reset && k == 0) ? cd : d + cd;
+    }
+};
+
+__device__ __forceinline__ PixelSums load_sums(const JobRecord& job, uint32_t px, bool aov) {
+    PixelSums s;
+    s.reset = job.flags & JOB_RESET;
+    s.aov = aov;
+    s.moments = job.moment_sum != nullptr;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s.b = s.reset ? zero : job.beauty_sum[px];
+    s.a = zero;
+    s.nn = zero;
+    s.d = 0.0f;
+    s.m = (s.moments && !s.reset) ? job.moment_sum[px] : 0.0f;
+    if (aov && !s.reset) {
+        s.a = job.albedo_sum[px];
+        s.nn = job.normal_sum[px];
+        s.d = job.depth_sum[px];
+    }
+    return s;
+}
+
+__device__ __forceinline__ void store_sums(const JobRecord& job, uint32_t px, const PixelSums& s) {
+    job.beauty_sum[px] = s.b;
+    if (s.moments) job.moment_sum[px] = s.m;
+    if (s.aov) {
+        job.albedo_sum[px] = s.a;
+        job.normal_sum[px] = s.nn;
+        job.depth_sum[px] = s.d;
+    }
+}
+
+__device__ __forceinline__ uint32_t job_of(const JobRecord* __restrict__ jobs, uint32_t job_count, uint32_t i) {
+    uint32_t lo = 0, hi = job_count;
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].pixel_base <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Samples staged per pass of the accumulate's LDS path.
+constexpr uint32_t ACC_CHUNK = 8;
+
 // One thread per view pixel of the wave: fold the job's samples into its
-// film's sums in sample order, reading the sums once.
+// film's sums in sample order, reading the sums once. A block takes BLOCK
+// consecutive band-order pixels. When they all lie in one job's full slot
+// groups (every tile of a one-job wave but its last), their slots are one
+// contiguous span, and the block stages them through LDS ACC_CHUNK samples
+// at a time: coalesced loads of each group's run of slots, then each
+// thread folds its own pixel's samples from LDS in sample order. Other
+// tiles fold straight from the slots, one thread per pixel.
 __global__ __launch_bounds__(BLOCK) void k_accumulate(const JobRecord* __restrict__ jobs, uint32_t job_count,
                                                       WaveTargets tg, uint32_t total_pixels) {
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total_pixels; i += gridDim.x * BLOCK) {
-        uint32_t lo = 0, hi = job_count;
-        while (lo + 1 < hi) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (jobs[mid].pixel_base <= i) lo = mid;
-            else hi = mid;
+    __shared__ float4 stage[BLOCK * (ACC_CHUNK + 1)];
+    for (uint32_t tile = blockIdx.x * BLOCK; tile < total_pixels; tile += gridDim.x * BLOCK) {
+        const JobRecord& tj = jobs[job_of(jobs, job_count, tile)];  // uniform per block
+        const uint32_t index0 = tile - tj.pixel_base;
+        const bool staged = tj.beauty_sum != nullptr && index0 % SLOT_GROUP == 0 &&
+                            index0 + BLOCK <= (tj.pixels / SLOT_GROUP) * SLOT_GROUP;
+        if (staged) {
+            const JobRecord& job = tj;
+            const uint32_t index = index0 + threadIdx.x;
+            const uint32_t px = band_pixel(index, job.width, job.height);
+            const bool aov = tg.albedo != nullptr && job.albedo_sum != nullptr;
+            PixelSums sums = load_sums(job, px, aov);
+            const uint32_t S = job.samples;
+            const uint32_t tile_slot = job.slot_base + index0 * S;  // the tile's groups, end to end
+            for (uint32_t k0 = 0; k0 < S; k0 += ACC_CHUNK) {
+                const uint32_t c = S - k0 < ACC_CHUNK ? S - k0 : ACC_CHUNK;
+                const uint32_t run = c * SLOT_GROUP;  // a group's slots for samples k0 .. k0 + c - 1
+#pragma unroll
+                for (uint32_t target = 0; target < 4; ++target) {
+                    if (target > 0 && !aov) break;
+                    auto load = [&](uint32_t slot) {
+                        if (target == 0) return tg.radiance[slot];
+                        if (target == 1) return tg.albedo[slot];
+                        if (target == 2) return tg.normal[slot];
+                        return make_float4(tg.depth[slot], 0.0f, 0.0f, 0.0f);
+                    };
+                    // Element e of the chunk: group e / run, offset e % run in the group's run.
+                    auto place = [&](uint32_t e, uint32_t q, uint32_t off) -> float4& {
+                        return stage[(q * SLOT_GROUP + off % SLOT_GROUP) * (ACC_CHUNK + 1) + off / SLOT_GROUP];
+                    };
+                    if (c == ACC_CHUNK) {
+                        // Every load of the pass in flight before the first LDS store.
+                        constexpr uint32_t RUN = ACC_CHUNK * SLOT_GROUP;
+                        float4 v[ACC_CHUNK];
+#pragma unroll
+                        for (uint32_t j = 0; j < ACC_CHUNK; ++j) {
+                            const uint32_t e = threadIdx.x + j * BLOCK, q = e / RUN, off = e % RUN;
+                            v[j] = load(tile_slot + q * SLOT_GROUP * S + k0 * SLOT_GROUP + off);
+                        }
+#pragma unroll
+                        for (uint32_t j = 0; j < ACC_CHUNK; ++j) {
+                            const uint32_t e = threadIdx.x + j * BLOCK, q = e / RUN, off = e % RUN;
+                            place(e, q, off) = v[j];
+                        }
+                    } else {
+                        for (uint32_t e = threadIdx.x; e < c * BLOCK; e += BLOCK) {
+                            const uint32_t q = e / run, off = e - q * run;
+                            place(e, q, off) = load(tile_slot + q * SLOT_GROUP * S + k0 * SLOT_GROUP + off);
+                        }
+                    }
+                    __syncthreads();
+                    const float4* mine = stage + threadIdx.x * (ACC_CHUNK + 1);
+                    for (uint32_t kk = 0; kk < c; ++kk) {
+                        if (target == 0) sums.radiance(k0 + kk, mine[kk]);
+                        else if (target == 1) sums.albedo(k0 + kk, mine[kk]);
+                        else if (target == 2) sums.normal(k0 + kk, mine[kk]);
+                        else sums.depth(k0 + kk, mine[kk].x);
+                    }
+                    __syncthreads();
+                }
+            }
+            store_sums(job, px, sums);
+            continue;
         }
-        const JobRecord& job = jobs[lo];
+        const uint32_t i = tile + threadIdx.x;
+        if (i >= total_pixels) continue;
+        const JobRecord& job = jobs[job_of(jobs, job_count, i)];
         if (!job.beauty_sum) continue;
-        // i walks the view in slot order (the slot reads coalesce); px is
-        // the film pixel that slot order index holds.
+        // px is the film pixel that band-order index holds; the pixel's
+        // slots are job_slot(index, k) = slot0 + k * stride.
         const uint32_t index = i - job.pixel_base;
         const uint32_t px = band_pixel(index, job.width, job.height);
-        const bool reset = job.flags & JOB_RESET;
-        float4 b = reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : job.beauty_sum[px];
         const bool aov = tg.albedo != nullptr && job.albedo_sum != nullptr;
-        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nn = a;
-        float d = 0.0f;
-        float m = (job.moment_sum && !reset) ? job.moment_sum[px] : 0.0f;
-        if (aov && !reset) {
-            a = job.albedo_sum[px];
-            nn = job.normal_sum[px];
-            d = job.depth_sum[px];
-        }
+        PixelSums sums = load_sums(job, px, aov);
+        const uint32_t slot0 = job.slot_base + job_slot(index, 0, job.pixels, job.samples);
+        const uint32_t stride = job_slot(index, 1, job.pixels, job.samples) - job_slot(index, 0, job.pixels, job.samples);
         for (uint32_t k = 0; k < job.samples; ++k) {
-            const uint32_t slot = job.slot_base + k * job.pixels + index;
-            const bool first = reset && k == 0;
-            const float4 cb = guarded(tg.radiance[slot]);
-            b = first ? cb : add4(b, cb);
-            if (job.moment_sum) {
-                const float y = (cb.x * LUMA_R + cb.y * LUMA_G) + cb.z * LUMA_B;
-                m = first ? y * y : m + y * y;
-            }
+            const uint32_t slot = slot0 + k * stride;
+            sums.radiance(k, tg.radiance[slot]);
             if (aov) {
-                const float4 ca = guarded(tg.albedo[slot]);
-                a = first ? ca : add4(a, ca);
-                const float4 cn = guarded(tg.normal[slot]);
-                nn = first ? cn : add4(nn, cn);
-                float cd = tg.depth[slot];
-                if (isnan(cd)) cd = 0.0f;
-                d = first ? cd : d + cd;
+                sums.albedo(k, tg.albedo[slot]);
+                sums.normal(k, tg.normal[slot]);
+                sums.depth(k, tg.depth[slot]);
             }
         }
-        job.beauty_sum[px] = b;
-        if (job.moment_sum) job.moment_sum[px] = m;
-        if (aov) {
-            job.albedo_sum[px] = a;
-            job.normal_sum[px] = nn;
-            job.depth_sum[px] = d;
-        }
+        store_sums(job, px, sums);
     }
 }
 
