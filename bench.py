@@ -146,7 +146,8 @@ def timed_launches(name, steps, capacity, tail=0):
     depth = cfg["max_bounces"]
     t = tail_split(depth, tail)
     traced = waves * min(depth, t + 1)
-    return {"raygen": waves, "trace": traced, "shade_miss": waves * t, "shade_surface": waves * t,
+    # bounce 0's misses are shaded by its trace launch
+    return {"raygen": waves, "trace": traced, "shade_miss": waves * max(0, t - 1), "shade_surface": waves * t,
             "accumulate": waves, "tail": waves if t < depth else 0}
 
 
@@ -175,7 +176,8 @@ def load_traffic(key):
 def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
     """Per-kernel algorithmic bytes of the counted waves, bounce by bounce; a
     node fetch counts its record size (64 bytes 4-wide, 128 bytes 8-wide).
-    With a tail after bounce t, trace holds bounces 0..t, shade_miss and
+    With a tail after bounce t, trace holds bounces 0..t (and the shading of
+    bounce 0's misses, which its launch does), shade_miss bounces 1..t-1,
     shade_surface bounces 0..t-1, and the tail kernel every later trace and
     all shading from bounce t on. `by_bounce[b]` = ray nodes, ray triangles,
     shadow nodes, shadow triangles of bounce b's trace launch (whose shadow
@@ -196,9 +198,9 @@ def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
     out = {
         "raygen": pixels * 72,
         "accumulate": pixels * (BYTES["camera_path_aov"] if aov else 48),
-        "trace": sum(traced[: t + 1]),
+        "trace": sum(traced[: t + 1]) + miss[0],
         "shade_surface": sum(surface[:t]),
-        "shade_miss": sum(miss[:t]),
+        "shade_miss": sum(miss[1:t]),
     }
     if t < depth:
         out["tail"] = sum(traced[t + 1:]) + sum(surface[t:]) + sum(miss[t:])

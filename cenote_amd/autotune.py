@@ -33,11 +33,12 @@ SHADE_VARIANTS = {
 
 
 def candidate_shapes(compute_units: int, trace_per_cu=(16, 32, 64), variants=tuple(SHADE_VARIANTS), widths=(4, 8),
-                     tails=(0,)):
+                     tails=(_abi.DEFAULT_TAIL_BOUNCE,)):
     """The default search grid: trace workgroups per CU x shade variants x
     BVH width (the 4- or 8-wide tree a scene carries) x tail bounce (0: the
-    wavefront stages throughout), with each shade variant's grid capped at
-    2048 lanes per CU (4096 for the 256-lane ones)."""
+    wavefront stages throughout; by default only the library's own), with
+    each shade variant's grid capped at 2048 lanes per CU (4096 for the
+    256-lane ones)."""
     for tpc, variant, width, tail in itertools.product(trace_per_cu, variants, widths, tails):
         block = SHADE_VARIANTS[variant]
         yield {
