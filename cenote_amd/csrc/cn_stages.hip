@@ -413,11 +413,12 @@ struct ShadowRecord {
 // as far as the connection calls `send_shadow(push, record)` (push false
 // when there is nothing to connect), so a sink may be a wave-wide claim.
 // Returns true when the path goes on (its next ray and throughput are in
-// the path records).
+// the path records); `below` then tells whether that ray leaves from below
+// the surface (transmitted).
 template <typename SendShadow>
 __device__ __forceinline__ bool shade_path(const SceneView& s, const PathPool& p, const WaveTargets& tg,
                                            const float* tables, uint32_t path, uint32_t base, uint32_t bounce,
-                                           uint32_t light_sampling, SendShadow&& send_shadow) {
+                                           uint32_t light_sampling, SendShadow&& send_shadow, bool& below) {
     const bool aovOn = tg.albedo != nullptr;
     bool push_ray = false;
     const uint32_t pixel = base + path;  // the path's target slot
@@ -577,6 +578,7 @@ __device__ __forceinline__ bool shade_path(const SceneView& s, const PathPool& p
                 medium = medium == hit.x ? MEDIUM_NONE : hit.x;
             }
             const bool transmitted = scatter.wi.z < 0.0f;
+            below = transmitted;
             p.origin[path] = F4(transmitted ? spawn.below : spawn.position, origin4.w);
             p.direction[path] = F4(wiWorld, 0.0f);
             p.throughput[path] = F4(scattered, scatter.pdf);
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
             valid = path != 0xffffffffu;
             if (!valid) path = 0;
         }
-        bool push_ray = false;
+        bool push_ray = false, below = false;
         if (valid) {
             push_ray = shade_path(s, p, tg, tables, path, base, bounce, light_sampling,
                                   [&](bool push, const ShadowRecord& r) {
@@ -671,14 +673,50 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
                                           p.shadow_contrib[ss] = F4(r.contrib, __uint_as_float(r.instance));
                                           p.shadow_prim[ss] = r.primitive;
                                       }
-                                  });
+                                  },
+                                  below);
         }
         if constexpr (SORT) {
             // The next bounce's rays go out in the hits' queue order, not in
             // material order: consecutive lanes of the next kernels then
-            // read the records of nearby paths.
+            // read the records of nearby paths. The block's rays are then
+            // stably partitioned, those leaving from above the surface first
+            // and the transmitted ones after: more of a trace wave's rays
+            // are of one kind and walk similar numbers of nodes, while the
+            // records stay within the block's own paths. (Grouping by
+            // direction octant as well scattered the next misses' slots and
+            // measured slower.)
             __shared__ uint32_t pushed[SB];
-            pushed[sorted_home[threadIdx.x]] = push_ray ? path : 0xffffffffu;
+            __shared__ uint32_t wave_counts[SB / 64][2];
+            constexpr uint32_t BELOW = 1u << 31;  // paths use HIT_PATH_BITS < 31 bits
+            pushed[sorted_home[threadIdx.x]] = push_ray ? (path | (below ? BELOW : 0u)) : 0xffffffffu;
+            __syncthreads();
+            const uint32_t v = pushed[threadIdx.x];
+            const bool ray = v != 0xffffffffu;
+            const bool down = ray && (v & BELOW);
+            const bool up = ray && !down;
+            const uint64_t mu = __ballot(up), md = __ballot(down);
+            const uint32_t wave = threadIdx.x / 64;
+            const uint64_t lt = (1ull << __lane_id()) - 1ull;
+            if (__lane_id() == 0) {
+                wave_counts[wave][0] = __popcll(mu);
+                wave_counts[wave][1] = __popcll(md);
+            }
+            __syncthreads();
+            uint32_t before_up = 0, before_down = 0, total_up = 0, total_down = 0;
+            for (uint32_t k = 0; k < SB / 64; ++k) {
+                const uint32_t cu = wave_counts[k][0], cd = wave_counts[k][1];
+                before_up += k < wave ? cu : 0u;
+                before_down += k < wave ? cd : 0u;
+                total_up += cu;
+                total_down += cd;
+            }
+            const uint32_t pos = up ? before_up + __popcll(mu & lt)
+                                    : down ? total_up + before_down + __popcll(md & lt)
+                                           : total_up + total_down + (threadIdx.x - (before_up + before_down) -
+                                                                       __popcll((mu | md) & lt));
+            __syncthreads();  // every lane has read its entry
+            pushed[pos] = ray ? (v & ~BELOW) : 0xffffffffu;
             __syncthreads();
             path = pushed[threadIdx.x];
             push_ray = path != 0xffffffffu;
@@ -724,11 +762,13 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         for (uint32_t b = bounce;; ++b) {
             bool connect = false;
             ShadowRecord sh;
+            bool below = false;
             const bool next = shade_path(s, p, tg, s.tables, path, base, b, light_sampling,
                                          [&](bool push, const ShadowRecord& r) {
                                              connect = push;
                                              sh = r;
-                                         });
+                                         },
+                                         below);
             if (connect) {
                 // As the trace kernel takes a connection: a solid hit short
                 // of a triangle light occludes without the closest hit.
