@@ -40,7 +40,7 @@ DIRTY_ALL = 31
 DEFAULT_CAPACITY = 1 << 20
 DEFAULT_MAX_BOUNCES = 8
 POOL_CAPACITY = 1 << 27
-DEFAULT_TAIL_BOUNCE = 5
+DEFAULT_TAIL_BOUNCE = 6
 STATIC_MAX_PIXELS = 1 << 18
 
 f3 = C.c_float * 3

@@ -74,8 +74,8 @@ enum {
 /* The largest pool a renderer may ask for. */
 #define CN_POOL_CAPACITY_MAX (1u << 27)
 /* A new renderer's tail bounce (cn_launch_shape.tail_bounce): after bounce
- * 5's trace one launch finishes every live path. */
-#define CN_DEFAULT_TAIL_BOUNCE 5u
+ * 6's trace one launch finishes every live path. */
+#define CN_DEFAULT_TAIL_BOUNCE 6u
 
 /*
  * One instance's lowered OpenPBR surface, 144 bytes, field for field the
