@@ -284,7 +284,17 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             const v3 o = xyz(p.origin[path]);
             const v3 d = xyz(p.direction[path]);
             const uint2 key = p.key[path];
-            const TraceHit h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
+            TraceHit h;
+            if constexpr (CAMERA && !W8 && !COUNT) {
+                // Camera rays walk the 4-wide tree as a wave (trace_packet),
+                // on the wave's own rows of the stack array. The counting
+                // form keeps the per-ray walk, whose fetches the byte model
+                // counts.
+                lds_u32* wstack = (lds_u32*)lds_stack + (threadIdx.x / 64) * STACK_MAX;
+                h = trace_packet(s, o, d, mask, key.x, key.y, wstack, ray_nodes, ray_tris);
+            } else {
+                h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
+            }
             hit = h.hit;
             if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
             cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
@@ -812,7 +822,7 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
                                                    tris);
                         return occluded;
                     }
-                    closest_leaf(s, w, ref, MASK_ALL, key.x, key.y, b + 1, best, tris);
+                    closest_leaf(s, TrisPerLane{s.tris}, w, ref, MASK_ALL, key.x, key.y, b + 1, best, tris);
                     return false;
                 });
             }

@@ -360,6 +360,31 @@ __device__ __forceinline__ void leaf_range(uint32_t ref, uint32_t* first, uint32
     *count = ((ref >> 27) & 15u) + 1u;
 }
 
+// The tree and triangles read as constant memory (address space 4): a load
+// at a wave-uniform address then goes through the scalar unit, once per
+// wave. Records are copied word by word (the scalar loads merge them).
+typedef const __attribute__((address_space(4))) uint32_t const_u32;
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* array, uint32_t i) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    const_u32* src = (const_u32*)(array + i);
+    T v;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
+    return v;
+}
+// The scene's triangles as the leaf test reads them: per thread, or through
+// the scalar unit at a wave-uniform index.
+struct TrisPerLane {
+    const BvhTriangle* tris;
+    __device__ __forceinline__ BvhTriangle operator[](uint32_t i) const { return tris[i]; }
+};
+struct TrisUniform {
+    const BvhTriangle* tris;
+    __device__ __forceinline__ BvhTriangle operator[](uint32_t i) const { return load_uniform(tris, i); }
+};
+
 struct TraceHit {
     bool hit;
     float t;
@@ -377,13 +402,15 @@ struct TraceHit {
 // per-surface trials the walk needs no such bound, and without it the
 // result does not depend on the order. Equal-t ties go to the lower
 // (instance, primitive).
-__device__ __forceinline__ void closest_leaf(const SceneView& s, const WRay& w, uint32_t ref, uint32_t mask,
-                                             uint32_t pixel, uint32_t sample, uint32_t bounce, TraceHit& best,
-                                             uint32_t& tris_tested) {
+// (`tris` reads the scene's triangles: TrisPerLane or TrisUniform.)
+template <typename Tris>
+__device__ __forceinline__ void closest_leaf(const SceneView& s, Tris tris, const WRay& w, uint32_t ref,
+                                             uint32_t mask, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                                             TraceHit& best, uint32_t& tris_tested) {
     uint32_t first, count;
     leaf_range(ref, &first, &count);
     for (uint32_t k = 0; k < count; ++k) {
-        const BvhTriangle tri = s.tris[first + k];
+        const BvhTriangle tri = tris[first + k];
         ++tris_tested;
         const uint32_t flags = __float_as_uint(tri.v2.w);
         if (((flags >> 8) & mask) == 0) continue;
@@ -406,9 +433,106 @@ __device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d
     const WRay w = make_wray(o, d);
     TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
     walk(tree_of<W8>(s), r, best.t, st, nodes_visited, [&](uint32_t ref) {
-        closest_leaf(s, w, ref, mask, pixel, sample, bounce, best, tris_tested);
+        closest_leaf(s, TrisPerLane{s.tris}, w, ref, mask, pixel, sample, bounce, best, tris_tested);
         return false;
     });
+    return best;
+}
+
+// Camera rays of one wave walk the 4-wide tree together. The node to visit
+// and the stack are the wave's own (one value for all lanes, the stack in
+// the wave's LDS rows); each lane tests its ray against the node's four
+// children, and the wave descends into every child that some lane's ray
+// enters within its closest hit so far, nearest first by the entry
+// distances of the first lane that enters any. Each lane thus visits every
+// node its own walk would visit (and perhaps more), so its closest hit —
+// which does not depend on the visiting order — is the same. Rays through
+// one or two pixels enter nearly the same nodes, so the wave walks about
+// one ray's path and fetches each node and triangle once for all lanes,
+// with no per-lane stack or child sorting. `wstack` holds STACK_MAX
+// entries for this wave.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d, uint32_t mask, uint32_t pixel,
+                                                 uint32_t sample, lds_u32* wstack, uint32_t& nodes_visited,
+                                                 uint32_t& tris_tested) {
+    const SlabRay r = make_slab(o, d);
+    const WRay w = make_wray(o, d);
+    TraceHit best{false, 1e30f, 0u, 0u, 0.0f, 0.0f, 0u};
+    const uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;  // the stack's writer
+    uint32_t sp = 0;   // wave-uniform
+    uint32_t cur = 0;  // wave-uniform; the root is always an internal node
+    while (cur != NODE_DONE) {
+        if (cur & CHILD_LEAF) {
+            closest_leaf(s, TrisUniform{s.tris}, w, cur, mask, pixel, sample, 0, best, tris_tested);
+            cur = sp == 0 ? NODE_DONE : wave_uniform(wstack[--sp]);
+            continue;
+        }
+        ++nodes_visited;
+        const BvhNode4Q n = load_uniform(s.nodes, cur);
+        const float sx = n.step_x * r.ix;
+        const float sy = n.step_y * r.iy;
+        const float sz = n.step_z * r.iz;
+        const float bx0 = (n.ox - r.x) * r.ix, by0 = (n.oy - r.y) * r.iy, bz0 = (n.oz - r.z) * r.iz;
+        const uint32_t near_x = r.nx ? n.hi_x : n.lo_x, far_x = r.nx ? n.lo_x : n.hi_x;
+        const uint32_t near_y = r.ny ? n.hi_y : n.lo_y, far_y = r.ny ? n.lo_y : n.hi_y;
+        const uint32_t near_z = r.nz ? n.hi_z : n.lo_z, far_z = r.nz ? n.lo_z : n.hi_z;
+        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        uint64_t entered[4];
+        uint32_t key[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float ax = fmaf(plane(near_x, c), sx, bx0), bx = fmaf(plane(far_x, c), sx, bx0);
+            const float ay = fmaf(plane(near_y, c), sy, by0), by = fmaf(plane(far_y, c), sy, by0);
+            const float az = fmaf(plane(near_z, c), sz, bz0), bz = fmaf(plane(far_z, c), sz, bz0);
+            const float t0 = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+            const float t1 = fminf(fminf(bx, by), fminf(bz, best.t)) * 1.0000004f;
+            const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
+            entered[c] = __ballot(hit);
+            key[c] = hit ? __float_as_uint(t0) : 0x7f800000u;  // t0 >= 0: its bits order as the values
+        }
+        const uint64_t any = entered[0] | entered[1] | entered[2] | entered[3];
+        if (any == 0) {
+            cur = sp == 0 ? NODE_DONE : wave_uniform(wstack[--sp]);
+            continue;
+        }
+        // The children some lane enters, keyed by the first such lane's
+        // entry distances (those it does not enter after, absent ones last).
+        const uint32_t lead = __ffsll((unsigned long long)any) - 1;
+        uint32_t k[4], ref[4];
+        uint32_t nh = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool used = entered[c] != 0;
+            k[c] = used ? wave_uniform(__builtin_amdgcn_readlane(key[c], lead)) : 0xffffffffu;
+            ref[c] = used ? ch[c] : NODE_DONE;
+            nh += used ? 1u : 0u;
+        }
+#define CN_UCE(a, b)                                          \
+    do {                                                      \
+        const bool swap = k[a] > k[b];                        \
+        const uint32_t kk = swap ? k[b] : k[a], rr = swap ? ref[b] : ref[a]; \
+        k[b] = swap ? k[a] : k[b];                            \
+        ref[b] = swap ? ref[a] : ref[b];                      \
+        k[a] = kk;                                            \
+        ref[a] = rr;                                          \
+    } while (0)
+        CN_UCE(0, 1);
+        CN_UCE(2, 3);
+        CN_UCE(0, 2);
+        CN_UCE(1, 3);
+        CN_UCE(1, 2);
+#undef CN_UCE
+        // push the rest farthest first (a push past STACK_MAX is dropped, as
+        // the per-lane stack does; the builder's bound keeps it from happening)
+        for (uint32_t c = nh; c-- > 1;) {
+            if (sp < STACK_MAX) {
+                if (__lane_id() == leader) wstack[sp] = ref[c];
+                ++sp;
+            }
+        }
+        cur = wave_uniform(ref[0]);
+    }
     return best;
 }
 
