@@ -208,7 +208,8 @@ def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
     return out, totals
 
 
-def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bvh_width=0, tail_bounce=None):
+def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bvh_width=0, tail_bounce=None,
+                  shape=None):
     """`steps` full frames of workload `name`, each a reset film and the
     workload's spp, stream-ordered with one synchronize at the end."""
     import cenote_amd as cn
@@ -229,6 +230,8 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
         renderer.shape = {"bvh_width": bvh_width}
     if tail_bounce is not None:
         renderer.shape = {"tail_bounce": tail_bounce}
+    if shape:
+        renderer.shape = shape
     # the tree the traversal walks (a scene without an 8-wide tree is walked 4-wide)
     walked = 8 if renderer.shape["bvh_width"] == 8 and info.bvh8_node_count else 4
     film = cn.Film(ctx, width, height)
@@ -690,6 +693,8 @@ def main():
     ap.add_argument("--tail-bounce", type=int, default=None,
                     help="the bounce after whose trace one launch finishes every live path (0: none; "
                          "default: the library's)")
+    ap.add_argument("--shape", type=json.loads, default=None,
+                    help='launch-shape fields to set, as JSON (e.g. \'{"shade_variant": 3}\')')
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (the PMC byte-counter calibration)")
     ap.add_argument("--valu-calibration", action="store_true",
@@ -745,7 +750,7 @@ def main():
 
     data = {args.workload: workloads.build(args.workload)}
     main_run = measure_plain(ctx, args.workload, args.steps, args.warmup, args.capacity, data[args.workload],
-                             bvh_width=args.bvh_width, tail_bounce=args.tail_bounce,
+                             bvh_width=args.bvh_width, tail_bounce=args.tail_bounce, shape=args.shape,
                              cpu=not args.no_cpu_baseline)
     result = {
         "metric": METRIC,
@@ -776,7 +781,7 @@ def main():
                 continue
             d = workloads.build(other)
             run = measure_plain(ctx, other, args.steps, args.warmup, args.capacity, d, bvh_width=args.bvh_width,
-                                tail_bounce=args.tail_bounce)
+                                tail_bounce=args.tail_bounce, shape=args.shape)
             beside[other] = {k: run[k] for k in ("value", "unit", "ms_per_frame", "frames", "grays_per_s",
                                                  "camera_hit_share", "kernel_ms_per_frame", "roofline", "config")}
             if other in ("chart", "cornell"):
