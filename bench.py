@@ -173,13 +173,15 @@ def load_traffic(key):
     return entry if isinstance(entry, dict) else None
 
 
-def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
+def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0, counter_lines=0):
     """Per-kernel algorithmic bytes of the counted waves, bounce by bounce; a
     node fetch counts its record size (64 bytes 4-wide, 128 bytes 8-wide).
-    With a tail after bounce t, trace holds bounces 0..t (and the shading of
-    bounce 0's misses, which its launch does), shade_miss bounces 1..t-1,
-    shade_surface bounces 0..t-1, and the tail kernel every later trace and
-    all shading from bounce t on. `by_bounce[b]` = ray nodes, ray triangles,
+    With a tail after bounce t, trace holds bounces 0..t (and the camera
+    paths' generation and bounce 0's miss shading, which its bounce-0 launch
+    does: the reference's raygen stage, 72 bytes per path), shade_miss
+    bounces 1..t-1, shade_surface bounces 0..t-1, and the tail kernel every
+    later trace and all shading from bounce t on. raygen itself only resets
+    the range's queue counters (128 bytes a counter line). `by_bounce[b]` = ray nodes, ray triangles,
     shadow nodes, shadow triangles of bounce b's trace launch (whose shadow
     rays are bounce b-1's connections)."""
     node = 128 if width == 8 else info.bvh_node_bytes
@@ -196,9 +198,9 @@ def algorithmic_bytes(counts, by_bounce, info, pixels, aov, width=4, tail=0):
         miss.append(BYTES["miss"] * int(counts["misses"][b]))
     t = tail_split(depth, tail)
     out = {
-        "raygen": pixels * 72,
+        "raygen": counter_lines * 128,
         "accumulate": pixels * (BYTES["camera_path_aov"] if aov else 48),
-        "trace": sum(traced[: t + 1]) + miss[0],
+        "trace": sum(traced[: t + 1]) + miss[0] + pixels * 72,
         "shade_surface": sum(surface[:t]),
         "shade_miss": sum(miss[1:t]),
     }
@@ -271,7 +273,9 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
     times = renderer.kernel_times()
     renderer.set_profiling(False)
 
-    frame_bytes, totals = algorithmic_bytes(counts, by_bounce, info, pixels * spp, aov, walked, tail)
+    # the ranges' counter lines: 4 queues x 8 sub-queues per bounce, plus a spare bounce
+    lines = (depth + 1) * 4 * 8 * waves_per_frame(width, height, spp, renderer.capacity)
+    frame_bytes, totals = algorithmic_bytes(counts, by_bounce, info, pixels * spp, aov, walked, tail, lines)
     value = pixels * spp * steps / elapsed / 1e6
     grays = (totals["rays"] + totals["shadows"]) * steps / elapsed / 1e9
     kernel_ms = {k: round(v[0] / steps, 4) for k, v in times.items()}

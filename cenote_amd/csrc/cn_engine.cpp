@@ -267,12 +267,12 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
             // counter sets are reused past MAX_RANGES (only the per-wave
             // diagnostics need them distinct).
             uint32_t* counters = r->counters.ptr + (k % MAX_RANGES) * per;
-            CN_HIP(launch_raygen(r->stream, dev + rg.g0, uint32_t(rg.g1 - rg.g0), pool, tg, counters,
-                                 counter_lines_per_range(r->max_bounces), rg.base, rg.count, r->profiler.get()));
+            CN_HIP(launch_raygen(r->stream, counters, counter_lines_per_range(r->max_bounces), r->profiler.get()));
             for (uint32_t b = 0; b < rg.rounds; ++b) {
                 const bool tail = b == tail_bounce;
-                CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, pool, tg, counters, rg.base,
-                                     rg.count, b, r->light_sampling, r->counting, tail, r->profiler.get()));
+                CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, pool, tg, counters, dev + rg.g0,
+                                     uint32_t(rg.g1 - rg.g0), rg.base, rg.count, b, r->light_sampling, r->counting,
+                                     tail, r->profiler.get()));
                 if (tail) break;
             }
         }

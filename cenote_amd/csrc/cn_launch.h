@@ -67,17 +67,17 @@ struct Profiler {
     virtual ~Profiler() = default;
 };
 
-// Camera paths for slots [base, base + count) of a wave whose jobs (sorted
-// by slot base) are `jobs[0 .. job_count)`.
-hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_lines, uint32_t base,
-                         uint32_t count, Profiler* prof);
+// Starts a range of a wave: resets its queue counters.
+hipError_t launch_raygen(hipStream_t stream, uint32_t* counters, uint32_t counter_lines, Profiler* prof);
 
 // One bounce: trace, then shade_miss and shade_surface — or, with `tail`,
-// trace and then the tail launch that finishes every live path.
+// trace and then the tail launch that finishes every live path. Bounce 0's
+// trace generates the camera paths of slots [base, base + range_count) of a
+// wave whose jobs (sorted by slot base) are `jobs[0 .. job_count)`.
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t light_sampling, int counting, bool tail, Profiler* prof);
+                         const WaveTargets& tg, uint32_t* counters, const JobRecord* jobs, uint32_t job_count,
+                         uint32_t base, uint32_t range_count, uint32_t bounce, uint32_t light_sampling, int counting,
+                         bool tail, Profiler* prof);
 
 // Fold every job's slots into its film sums (jobs sorted by pixel base;
 // total_pixels is the sum of the jobs' view pixels).

@@ -150,19 +150,17 @@ __device__ __forceinline__ void camera_ray(const CameraView& cam, uint32_t width
     *cos_forward = dot(d, normalize(forward));
 }
 
-// One thread per slot of the range: find the slot's job (jobs sorted by
-// slot base), derive its pixel and sample, and start the camera path. Also
-// resets the range's queue counters (one word per 128-byte counter line):
-// raygen owns every slot exactly once. The reference's Pass::Fill
-// (wavefront.rs:733-759) zero-fills the slot's accumulators; here the
-// bounce-0 kernels store their first values instead (put_pixel).
-__global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ jobs, uint32_t job_count, PathPool p,
-                                                  WaveTargets tg, uint32_t* counters, uint32_t counter_lines,
-                                                  uint32_t base, uint32_t count) {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < counter_lines) counters[i * COUNTER_STRIDE] = 0;
-    if (i >= count) return;
-    const uint32_t slot = base + i;
+// The camera path of wave slot `slot`: its job (jobs sorted by slot base),
+// pixel and sample, and the camera ray (raygen.slang:64-112).
+struct CameraPath {
+    v3 o, d;
+    float cos_forward;
+    uint2 key;             // (pixel, sample)
+    uint32_t max_bounces;  // the job's depth cap
+};
+
+__device__ __forceinline__ CameraPath camera_path(const JobRecord* __restrict__ jobs, uint32_t job_count,
+                                                  uint32_t slot) {
     uint32_t lo = 0, hi = job_count;
     while (lo + 1 < hi) {
         const uint32_t mid = (lo + hi) / 2;
@@ -172,16 +170,22 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
     const JobRecord& job = jobs[lo];
     uint32_t index, k;
     job_unslot(slot - job.slot_base, job.pixels, job.samples, &index, &k);
-    const uint32_t pixel = band_pixel(index, job.width, job.height);
-    const uint32_t sample = job.first_sample + k;
-    v3 o, d;
-    float cf;
-    camera_ray(job.cam, job.width, job.height, pixel, sample, &o, &d, &cf);
-    p.origin[i] = F4(o, __uint_as_float(job.max_bounces));  // w: the path's depth cap
-    p.direction[i] = F4(d, cf);
-    // A camera path's throughput, (1, 1, 1) with no previous pdf, is not
-    // stored: the bounce-0 kernels use camera_throughput().
-    p.key[i] = make_uint2(pixel, sample);
+    CameraPath c;
+    c.key = make_uint2(band_pixel(index, job.width, job.height), job.first_sample + k);
+    camera_ray(job.cam, job.width, job.height, c.key.x, c.key.y, &c.o, &c.d, &c.cos_forward);
+    c.max_bounces = job.max_bounces;
+    return c;
+}
+
+// Starts a range: resets its queue counters (one word per 128-byte counter
+// line). The camera paths themselves are generated where bounce 0 traces
+// them (k_trace), which stores the records of the paths that hit; the
+// reference's raygen pass (raygen.slang:64-112, with Pass::Fill's zero-fill
+// of the slot's accumulators, wavefront.rs:733-759) has no separate pass
+// here: the bounce-0 kernels store each slot's first values (put_pixel).
+__global__ __launch_bounds__(BLOCK) void k_raygen(uint32_t* counters, uint32_t counter_lines) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < counter_lines) counters[i * COUNTER_STRIDE] = 0;
 }
 
 // --- shade_miss (shade_miss.slang:27-75) ----------------------------------------------
@@ -189,11 +193,8 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(const JobRecord* __restrict__ 
 // One path whose ray at `bounce` escaped: the environment's radiance, MIS
 // weighted against next-event estimation, lands in the path's slot, and a
 // camera path's AOVs take the miss values.
-__device__ __forceinline__ void miss_path(const SceneView& s, const PathPool& p, const WaveTargets& tg, uint32_t path,
-                                          uint32_t base, uint32_t bounce, uint32_t light_sampling) {
-    const uint32_t pixel = base + path;
-    const v3 direction = xyz(p.direction[path]);
-    const float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
+__device__ __forceinline__ void miss_shade(const SceneView& s, const WaveTargets& tg, uint32_t pixel, v3 direction,
+                                           float4 thr, uint32_t bounce, uint32_t light_sampling) {
     float weight = 1.0f;
     if (thr.w > 0.0f) {
         if (light_sampling == CN_LIGHT_SAMPLING_NEE) {
@@ -222,6 +223,12 @@ __device__ __forceinline__ void miss_path(const SceneView& s, const PathPool& p,
     }
     const v3 contribution = (xyz(thr) * env_radiance(s, direction)) * weight;
     put_pixel(tg.radiance, pixel, contribution, 1.0f, bounce == 0);
+}
+
+__device__ __forceinline__ void miss_path(const SceneView& s, const PathPool& p, const WaveTargets& tg, uint32_t path,
+                                          uint32_t base, uint32_t bounce, uint32_t light_sampling) {
+    const float4 thr = bounce == 0 ? camera_throughput() : p.throughput[path];
+    miss_shade(s, tg, base + path, xyz(p.direction[path]), thr, bounce, light_sampling);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
@@ -263,6 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_miss(SceneView s, PathPool p, W
 // registers).
 template <bool COUNT, bool W8, bool CAMERA>
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(W8 ? TRACE8_WAVES : 8))) void k_trace(SceneView s, PathPool p, WaveTargets tg, uint32_t* counters,
+                                                           const JobRecord* __restrict__ jobs, uint32_t job_count,
                                                            uint32_t base, uint32_t range_count, uint32_t bounce,
                                                            uint32_t light_sampling) {
     __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
@@ -280,10 +288,22 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         uint32_t path = 0, cls = 0;
         bool hit = false;
         if (is_ray) {
-            path = CAMERA ? i : rays[queue_slot(counters, bounce, Q_RAY, i, p.sub_capacity)];
-            const v3 o = xyz(p.origin[path]);
-            const v3 d = xyz(p.direction[path]);
-            const uint2 key = p.key[path];
+            v3 o, d;
+            uint2 key;
+            CameraPath cam;
+            if constexpr (CAMERA) {
+                // bounce 0 generates the camera path of slot base + i
+                path = i;
+                cam = camera_path(jobs, job_count, base + i);
+                o = cam.o;
+                d = cam.d;
+                key = cam.key;
+            } else {
+                path = rays[queue_slot(counters, bounce, Q_RAY, i, p.sub_capacity)];
+                o = xyz(p.origin[path]);
+                d = xyz(p.direction[path]);
+                key = p.key[path];
+            }
             TraceHit h;
             if constexpr (CAMERA && !W8 && !COUNT) {
                 // Camera rays walk the 4-wide tree as a wave (trace_packet),
@@ -296,7 +316,21 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
                 h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);
             }
             hit = h.hit;
-            if (hit) p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+            if (hit) {
+                p.hit[path] = make_uint4(h.inst, h.prim, __float_as_uint(h.b1), __float_as_uint(h.b2));
+                if constexpr (CAMERA) {
+                    // the path's records, for its shading and later bounces
+                    p.origin[path] = F4(o, __uint_as_float(cam.max_bounces));  // w: the path's depth cap
+                    p.direction[path] = F4(d, cam.cos_forward);
+                    p.key[path] = key;
+                }
+            } else if constexpr (CAMERA) {
+                // A camera ray that escapes is shaded here: no other item of
+                // this launch adds to its slot (bounce 0 carries no
+                // connections). A camera path's throughput, (1, 1, 1) with no
+                // previous pdf, is never stored.
+                miss_shade(s, tg, base + path, d, camera_throughput(), 0, light_sampling);
+            }
             cls = (h.flags >> TRI_CLASS_SHIFT) & (MATERIAL_CLASSES - 1);
         } else if (i < n) {
             const uint32_t j = queue_slot(counters, bounce - 1, Q_SHADOW, i - n_ray, p.sub_capacity);
@@ -319,13 +353,8 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             if (is_ray) {
                 const uint32_t at = split * p.sub_capacity;
                 if (hit) p.hit_queue[at + hs] = path | (cls << HIT_PATH_BITS);
-                else if (!CAMERA) p.miss_queue[at + ms] = path;  // bounce 0's are counted, and shaded below
+                else if (!CAMERA) p.miss_queue[at + ms] = path;  // bounce 0's are counted, and shaded above
             }
-        }
-        // A camera ray that escaped is shaded here: no other item of this
-        // launch adds to its slot (bounce 0 carries no connections).
-        if constexpr (CAMERA) {
-            if (is_ray && !hit) miss_path(s, p, tg, path, base, 0, light_sampling);
         }
     }
     if constexpr (COUNT) {
@@ -1135,21 +1164,18 @@ static uint32_t grid_for(uint32_t items, uint32_t cap) {
     return blocks < cap ? blocks : cap;
 }
 
-hipError_t launch_raygen(hipStream_t stream, const JobRecord* jobs, uint32_t job_count, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t counter_lines, uint32_t base,
-                         uint32_t count, Profiler* prof) {
-    const uint32_t items = count > counter_lines ? count : counter_lines;
+hipError_t launch_raygen(hipStream_t stream, uint32_t* counters, uint32_t counter_lines, Profiler* prof) {
     if (prof) prof->begin(K_RAYGEN);
-    hipLaunchKernelGGL(k_raygen, dim3((items + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, jobs, job_count, p, tg,
-                       counters, counter_lines, base, count);
+    hipLaunchKernelGGL(k_raygen, dim3((counter_lines + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, counters,
+                       counter_lines);
     if (prof) prof->end(K_RAYGEN);
     return hipGetLastError();
 }
 
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
-                         const WaveTargets& tg, uint32_t* counters, uint32_t base, uint32_t range_count,
-                         uint32_t bounce, uint32_t light_sampling, int counting, bool tail,
-                         Profiler* prof) {
+                         const WaveTargets& tg, uint32_t* counters, const JobRecord* jobs, uint32_t job_count,
+                         uint32_t base, uint32_t range_count, uint32_t bounce, uint32_t light_sampling, int counting,
+                         bool tail, Profiler* prof) {
     const uint32_t trace_grid = shape.trace_blocks;
     const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
     // Bounce b's rays and bounce b-1's shadow rays share one launch; the
@@ -1158,7 +1184,7 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
     const bool wide = shape.bvh_width == 8 && s.nodes8 != nullptr;
 #define CN_TRACE(COUNT, W8, CAMERA)                                                                                 \
     hipLaunchKernelGGL((k_trace<COUNT, W8, CAMERA>), dim3(trace_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg,   \
-                       counters, base, range_count, bounce, light_sampling)
+                       counters, jobs, job_count, base, range_count, bounce, light_sampling)
 #define CN_TRACE_CAMERA(COUNT, W8)   \
     if (bounce == 0) CN_TRACE(COUNT, W8, true); \
     else CN_TRACE(COUNT, W8, false)
