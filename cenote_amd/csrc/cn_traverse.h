@@ -231,7 +231,11 @@ __device__ __forceinline__ void order_pair(float& ka, uint32_t& ca, float& kb, u
 // keeps the test conservative.
 __device__ __forceinline__ uint32_t visit(const BvhNode4Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                           float bound, Stack& st) {
-    const BvhNode4Q n = nodes[index];
+    // A 32-bit byte offset from the tree's base (a node index is below 2^26:
+    // leaves address at most 2^27 triangles, a node holds four children),
+    // so the fetch takes the base from scalar registers.
+    const BvhNode4Q n =
+        *reinterpret_cast<const BvhNode4Q*>(reinterpret_cast<const char*>(nodes) + index * uint32_t(sizeof(BvhNode4Q)));
     const float sx = n.step_x * r.ix;
     const float sy = n.step_y * r.iy;
     const float sz = n.step_z * r.iz;
@@ -272,7 +276,9 @@ __device__ __forceinline__ uint32_t visit(const BvhNode4Q* __restrict__ nodes, u
 // hit, visibility and transmittance do not depend on it.
 __device__ __forceinline__ uint32_t visit(const BvhNode8Q* __restrict__ nodes, uint32_t index, const SlabRay& r,
                                           float bound, Stack& st) {
-    const uint4* line = reinterpret_cast<const uint4*>(nodes + index);
+    // a 32-bit byte offset, as for the 4-wide nodes (8-wide indices are below 2^25)
+    const uint4* line =
+        reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(nodes) + index * uint32_t(sizeof(BvhNode8Q)));
     const uint4 h = line[0];
     const uint4 px = line[1];  // lo_x[0], lo_x[1], hi_x[0], hi_x[1]
     const uint4 py = line[2];
