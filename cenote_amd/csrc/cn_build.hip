@@ -1325,7 +1325,43 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_patch_flags(BvhTriangle* __rest
     if (inst < instances) tris[i].v2.w = __uint_as_float(instance_flags[inst]);
 }
 
+// One thread per node: each plane byte q of the quantized node becomes
+// q * step (exact), the origin and children carried over.
+__global__ __launch_bounds__(BUILD_BLOCK) void k_packet_nodes(const BvhNode4Q* __restrict__ in, uint32_t n,
+                                                              BvhNode4P* __restrict__ out) {
+    const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const BvhNode4Q q = in[i];
+    BvhNode4P p;
+    p.ox = q.ox;
+    p.oy = q.oy;
+    p.oz = q.oz;
+    p.pad = 0.0f;
+    for (int c = 0; c < 4; ++c) {
+        const int sh = 8 * c;
+        p.lo_x[c] = float((q.lo_x >> sh) & 0xffu) * q.step_x;
+        p.hi_x[c] = float((q.hi_x >> sh) & 0xffu) * q.step_x;
+        p.lo_y[c] = float((q.lo_y >> sh) & 0xffu) * q.step_y;
+        p.hi_y[c] = float((q.hi_y >> sh) & 0xffu) * q.step_y;
+        p.lo_z[c] = float((q.lo_z >> sh) & 0xffu) * q.step_z;
+        p.hi_z[c] = float((q.hi_z >> sh) & 0xffu) * q.step_z;
+    }
+    p.child[0] = q.child.x;
+    p.child[1] = q.child.y;
+    p.child[2] = q.child.z;
+    p.child[3] = q.child.w;
+    out[i] = p;
+}
+
 }  // namespace
+
+bool decode_packet_nodes(const BvhNode4Q* nodes, uint32_t count, BvhNode4P* out, std::string* error) {
+    if (count == 0) return true;
+    hipLaunchKernelGGL(k_packet_nodes, dim3(blocks_for(count)), dim3(BUILD_BLOCK), 0, nullptr, nodes, count, out);
+    BUILD_CHECK(hipGetLastError());
+    BUILD_CHECK(hipDeviceSynchronize());
+    return true;
+}
 
 bool patch_triangle_flags(BvhTriangle* tris, uint32_t count, const uint32_t* instance_flags, uint32_t instances,
                           std::string* error) {

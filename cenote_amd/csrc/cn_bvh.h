@@ -79,6 +79,10 @@ bool build_bvh_device(const std::vector<WorldTriangle>& input, uint32_t max_stac
 bool patch_triangle_flags(BvhTriangle* tris, uint32_t count, const uint32_t* instance_flags, uint32_t instances,
                           std::string* error);
 
+// The packet walk's 128-byte nodes (BvhNode4P) from the device's quantized
+// ones, node for node, on the device (the same boxes, planes decoded).
+bool decode_packet_nodes(const BvhNode4Q* nodes, uint32_t count, BvhNode4P* out, std::string* error);
+
 // The traversal's 64-byte nodes (BvhNode4Q) from the full-precision ones,
 // node for node: each node's grid starts at the low corner of its
 // children's union and steps by the smallest power of two that spans the

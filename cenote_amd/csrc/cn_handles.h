@@ -108,6 +108,7 @@ struct cn_scene {
     uint64_t serial = 0;  // unique per scene created in this process (graph-cache keys)
     cn_camera camera{};
     cn::DevBuf<cn::BvhNode4Q> nodes;        // what the traversal reads
+    cn::DevBuf<cn::BvhNode4P> pnodes;       // the same nodes decoded for camera-ray packets
     std::vector<cn::BvhNode4> host_nodes;    // the full-precision tree (cn_scene_bvh)
     cn::DevBuf<cn::BvhNode8Q> nodes8;       // the 8-wide tree over the same leaves (may be empty)
     std::vector<cn::BvhNode8Q> host_nodes8;

@@ -100,6 +100,25 @@ struct alignas(16) BvhNode4Q {
 };
 static_assert(sizeof(BvhNode4Q) == 64, "half a 128-byte line per node");
 
+// The same node as a wave of camera rays walks it (128 bytes, decoded from
+// the quantized node on the device): child c's plane on axis k at
+// origin_k + off_k[c], off = q * step exact in f32 (an 8-bit q times a
+// power of two), so a lane meets a plane with one fma. The wave fetches a
+// node once, through the scalar unit, for all its lanes, and picks the near
+// and far plane words there when its rays' direction signs agree.
+struct alignas(16) BvhNode4P {
+    float ox, oy, oz, pad;
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    uint32_t child[4];
+};
+static_assert(sizeof(BvhNode4P) == 128, "one 128-byte line per packet node");
+// The packet walk addresses the plane arrays by word offset.
+static_assert(offsetof(BvhNode4P, lo_x) == 4 * 4 && offsetof(BvhNode4P, hi_x) == 8 * 4 &&
+                  offsetof(BvhNode4P, lo_y) == 12 * 4 && offsetof(BvhNode4P, hi_y) == 16 * 4 &&
+                  offsetof(BvhNode4P, lo_z) == 20 * 4 && offsetof(BvhNode4P, hi_z) == 24 * 4 &&
+                  offsetof(BvhNode4P, child) == 28 * 4,
+              "packet node word layout");
+
 // The 8-wide traversal node: eight children on the same kind of per-node
 // grid, one 128-byte line. Byte c of each plane pair (lo_x[0] holds
 // children 0..3, lo_x[1] children 4..7) is child c's plane in grid steps;
@@ -167,6 +186,7 @@ static_assert(sizeof(cn_light_record) == 80, "lights.rs:115");
 struct SceneView {
     const BvhNode4Q* nodes;
     const BvhNode8Q* nodes8;  // the 8-wide tree over the same leaves (null when the scene has none)
+    const BvhNode4P* pnodes;  // `nodes` decoded for wave-packet walks, node for node
     const BvhTriangle* tris;
     const InstanceRecord* instances;
     const cn_material* materials;

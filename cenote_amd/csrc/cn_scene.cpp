@@ -219,6 +219,7 @@ void refresh_view(cn_scene* scene) {
     SceneView& v = scene->view;
     v.nodes = scene->nodes.ptr;
     v.nodes8 = scene->nodes8.ptr;  // null without an 8-wide tree
+    v.pnodes = scene->pnodes.ptr;
     v.tris = scene->tris.ptr;
     v.instances = scene->instances.ptr;
     v.materials = scene->materials.ptr;
@@ -298,6 +299,9 @@ int prep_scene(cn_context* ctx, const cn_scene_desc* desc, cn_scene* scene) {
         scene->nodes.adopt(dev.nodes, bvh.nodes.size());
         if (dev.nodes8) scene->nodes8.adopt(dev.nodes8, bvh.nodes8.size());
     }
+    CN_HIP(scene->pnodes.alloc(scene->nodes.count));
+    if (!decode_packet_nodes(scene->nodes.ptr, uint32_t(scene->nodes.count), scene->pnodes.ptr, &error))
+        return fail(CN_E_SCENE, "%s", error.c_str());
     CN_HIP(scene->instances.upload(records.data(), records.size()));
     CN_HIP(scene->materials.upload(materials.data(), materials.size()));
     CN_HIP(scene->positions.upload(positions.data(), positions.size()));

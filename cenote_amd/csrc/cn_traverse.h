@@ -10,6 +10,8 @@
 // lane of the wave64 holds a leaf, so leaf tests run with full SIMD lanes.
 #pragma once
 
+#include <type_traits>
+
 #include "cn_launch.h"
 #include "cn_math.h"
 #include "cn_records.h"
@@ -380,6 +382,28 @@ __device__ __forceinline__ T load_uniform(const T* array, uint32_t i) {
     for (uint32_t k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
     return v;
 }
+// A packet node (128 bytes at a wave-uniform address) in one round trip of
+// the scalar unit: two 64-byte loads from one base, waited for in the same
+// block, so no load is in flight when the registers are handed back.
+typedef uint32_t sgpr_x16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ BvhNode4P load_packet_node(const BvhNode4P* nodes, uint32_t index) {
+    const uint64_t at = reinterpret_cast<uint64_t>(nodes + index);
+    sgpr_x16 a, b;
+    asm volatile(
+        "s_load_dwordx16 %0, %2, 0x0\n\t"
+        "s_load_dwordx16 %1, %2, 0x40\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(a), "=&s"(b)
+        : "s"(at));
+    BvhNode4P n;
+    uint32_t* w = reinterpret_cast<uint32_t*>(&n);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        w[k] = a[k];
+        w[16 + k] = b[k];
+    }
+    return n;
+}
 // The scene's triangles as the leaf test reads them: per thread, or through
 // the scalar unit at a wave-uniform index.
 struct TrisPerLane {
@@ -455,15 +479,24 @@ __device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d
 // which does not depend on the visiting order — is the same. Rays through
 // one or two pixels enter nearly the same nodes, so the wave walks about
 // one ray's path and fetches each node and triangle once for all lanes,
-// with no per-lane stack or child sorting. `wstack` holds STACK_MAX
-// entries for this wave.
+// with no per-lane stack or child sorting. The wave reads the decoded
+// nodes (BvhNode4P): the quantized node's boxes with every plane as an f32
+// offset, met with one fma per plane — the per-lane walk's test rounded
+// once less, conservative within the same outward padding. `wstack` holds
+// STACK_MAX entries for this wave.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d, uint32_t mask, uint32_t pixel,
-                                                 uint32_t sample, lds_u32* wstack, uint32_t& nodes_visited,
-                                                 uint32_t& tris_tested) {
-    const SlabRay r = make_slab(o, d);
-    const WRay w = make_wray(o, d);
+// SIGNS: bit k set when every lane's direction is negative on axis k (its
+// near plane on that axis is the high one), or PACKET_MIXED_SIGNS when the
+// lanes differ on some axis and each orders every plane pair itself.
+constexpr int PACKET_MIXED_SIGNS = 8;
+template <int SIGNS>
+__device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const SlabRay& r, const WRay& w,
+                                                       uint32_t mask, uint32_t pixel, uint32_t sample,
+                                                       lds_u32* wstack, uint32_t& nodes_visited,
+                                                       uint32_t& tris_tested) {
+    constexpr bool MIXED = SIGNS == PACKET_MIXED_SIGNS;
+    constexpr bool NX = !MIXED && (SIGNS & 1), NY = !MIXED && (SIGNS & 2), NZ = !MIXED && (SIGNS & 4);
     TraceHit best{false, 1e30f, 0u, 0u, 0.0f, 0.0f, 0u};
     const uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;  // the stack's writer
     uint32_t sp = 0;   // wave-uniform
@@ -475,25 +508,33 @@ __device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d,
             continue;
         }
         ++nodes_visited;
-        const BvhNode4Q n = load_uniform(s.nodes, cur);
-        const float sx = n.step_x * r.ix;
-        const float sy = n.step_y * r.iy;
-        const float sz = n.step_z * r.iz;
+        const BvhNode4P n = load_packet_node(s.pnodes, cur);
         const float bx0 = (n.ox - r.x) * r.ix, by0 = (n.oy - r.y) * r.iy, bz0 = (n.oz - r.z) * r.iz;
-        const uint32_t near_x = r.nx ? n.hi_x : n.lo_x, far_x = r.nx ? n.lo_x : n.hi_x;
-        const uint32_t near_y = r.ny ? n.hi_y : n.lo_y, far_y = r.ny ? n.lo_y : n.hi_y;
-        const uint32_t near_z = r.nz ? n.hi_z : n.lo_z, far_z = r.nz ? n.lo_z : n.hi_z;
-        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        const float* near_x = NX ? n.hi_x : n.lo_x;
+        const float* far_x = NX ? n.lo_x : n.hi_x;
+        const float* near_y = NY ? n.hi_y : n.lo_y;
+        const float* far_y = NY ? n.lo_y : n.hi_y;
+        const float* near_z = NZ ? n.hi_z : n.lo_z;
+        const float* far_z = NZ ? n.lo_z : n.hi_z;
         uint64_t entered[4];
         uint32_t key[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float ax = fmaf(plane(near_x, c), sx, bx0), bx = fmaf(plane(far_x, c), sx, bx0);
-            const float ay = fmaf(plane(near_y, c), sy, by0), by = fmaf(plane(far_y, c), sy, by0);
-            const float az = fmaf(plane(near_z, c), sz, bz0), bz = fmaf(plane(far_z, c), sz, bz0);
+            float ax = fmaf(near_x[c], r.ix, bx0), bx = fmaf(far_x[c], r.ix, bx0);
+            float ay = fmaf(near_y[c], r.iy, by0), by = fmaf(far_y[c], r.iy, by0);
+            float az = fmaf(near_z[c], r.iz, bz0), bz = fmaf(far_z[c], r.iz, bz0);
+            if (MIXED) {
+                const float lx = ax, ly = ay, lz = az;
+                ax = fminf(lx, bx);
+                bx = fmaxf(lx, bx);
+                ay = fminf(ly, by);
+                by = fmaxf(ly, by);
+                az = fminf(lz, bz);
+                bz = fmaxf(lz, bz);
+            }
             const float t0 = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
             const float t1 = fminf(fminf(bx, by), fminf(bz, best.t)) * 1.0000004f;
-            const bool hit = t0 <= t1 && ch[c] != CHILD_EMPTY;
+            const bool hit = t0 <= t1 && n.child[c] != CHILD_EMPTY;
             entered[c] = __ballot(hit);
             key[c] = hit ? __float_as_uint(t0) : 0x7f800000u;  // t0 >= 0: its bits order as the values
         }
@@ -511,7 +552,7 @@ __device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d,
         for (int c = 0; c < 4; ++c) {
             const bool used = entered[c] != 0;
             k[c] = used ? wave_uniform(__builtin_amdgcn_readlane(key[c], lead)) : 0xffffffffu;
-            ref[c] = used ? ch[c] : NODE_DONE;
+            ref[c] = used ? n.child[c] : NODE_DONE;
             nh += used ? 1u : 0u;
         }
 #define CN_UCE(a, b)                                          \
@@ -540,6 +581,33 @@ __device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d,
         cur = wave_uniform(ref[0]);
     }
     return best;
+}
+
+__device__ __forceinline__ TraceHit trace_packet(const SceneView& s, v3 o, v3 d, uint32_t mask, uint32_t pixel,
+                                                 uint32_t sample, lds_u32* wstack, uint32_t& nodes_visited,
+                                                 uint32_t& tris_tested) {
+    const SlabRay r = make_slab(o, d);
+    const WRay w = make_wray(o, d);
+    // Whether every lane's ray points the same way on each axis (nearly
+    // always, for the samples of one or two pixels): then one plane of each
+    // pair is every lane's near plane, fixed for the whole walk.
+    const uint64_t live = __ballot(1);
+    const uint64_t sx = __ballot(r.nx), sy = __ballot(r.ny), sz = __ballot(r.nz);
+    const bool same = (sx == 0 || sx == live) && (sy == 0 || sy == live) && (sz == 0 || sz == live);
+    const int signs = same ? int(sx != 0) | int(sy != 0) << 1 | int(sz != 0) << 2 : PACKET_MIXED_SIGNS;
+    switch (signs) {
+        case 0: return trace_packet_signs<0>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 1: return trace_packet_signs<1>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 2: return trace_packet_signs<2>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 3: return trace_packet_signs<3>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 4: return trace_packet_signs<4>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 5: return trace_packet_signs<5>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 6: return trace_packet_signs<6>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        case 7: return trace_packet_signs<7>(s, r, w, mask, pixel, sample, wstack, nodes_visited, tris_tested);
+        default:
+            return trace_packet_signs<PACKET_MIXED_SIGNS>(s, r, w, mask, pixel, sample, wstack, nodes_visited,
+                                                          tris_tested);
+    }
 }
 
 // The shadow leaf test: returns true when a solid non-target hit nearer
