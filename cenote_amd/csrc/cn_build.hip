@@ -1326,7 +1326,9 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_patch_flags(BvhTriangle* __rest
 }
 
 // One thread per node: each plane byte q of the quantized node becomes
-// q * step (exact), the origin and children carried over.
+// q * step (exact), the origin and children carried over. An empty slot's
+// box is inverted to +inf .. -inf on every axis, so a walk that takes each
+// axis's near plane by the direction's sign never enters it.
 __global__ __launch_bounds__(BUILD_BLOCK) void k_packet_nodes(const BvhNode4Q* __restrict__ in, uint32_t n,
                                                               BvhNode4P* __restrict__ out) {
     const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
@@ -1337,19 +1339,19 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_packet_nodes(const BvhNode4Q* _
     p.oy = q.oy;
     p.oz = q.oz;
     p.pad = 0.0f;
+    const uint32_t child[4] = {q.child.x, q.child.y, q.child.z, q.child.w};
+    const float inf = __int_as_float(0x7f800000);
     for (int c = 0; c < 4; ++c) {
         const int sh = 8 * c;
-        p.lo_x[c] = float((q.lo_x >> sh) & 0xffu) * q.step_x;
-        p.hi_x[c] = float((q.hi_x >> sh) & 0xffu) * q.step_x;
-        p.lo_y[c] = float((q.lo_y >> sh) & 0xffu) * q.step_y;
-        p.hi_y[c] = float((q.hi_y >> sh) & 0xffu) * q.step_y;
-        p.lo_z[c] = float((q.lo_z >> sh) & 0xffu) * q.step_z;
-        p.hi_z[c] = float((q.hi_z >> sh) & 0xffu) * q.step_z;
+        const bool empty = child[c] == CHILD_EMPTY;
+        p.lo_x[c] = empty ? inf : float((q.lo_x >> sh) & 0xffu) * q.step_x;
+        p.hi_x[c] = empty ? -inf : float((q.hi_x >> sh) & 0xffu) * q.step_x;
+        p.lo_y[c] = empty ? inf : float((q.lo_y >> sh) & 0xffu) * q.step_y;
+        p.hi_y[c] = empty ? -inf : float((q.hi_y >> sh) & 0xffu) * q.step_y;
+        p.lo_z[c] = empty ? inf : float((q.lo_z >> sh) & 0xffu) * q.step_z;
+        p.hi_z[c] = empty ? -inf : float((q.hi_z >> sh) & 0xffu) * q.step_z;
+        p.child[c] = child[c];
     }
-    p.child[0] = q.child.x;
-    p.child[1] = q.child.y;
-    p.child[2] = q.child.z;
-    p.child[3] = q.child.w;
     out[i] = p;
 }
 

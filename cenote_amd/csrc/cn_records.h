@@ -105,7 +105,8 @@ static_assert(sizeof(BvhNode4Q) == 64, "half a 128-byte line per node");
 // origin_k + off_k[c], off = q * step exact in f32 (an 8-bit q times a
 // power of two), so a lane meets a plane with one fma. The wave fetches a
 // node once, through the scalar unit, for all its lanes, and picks the near
-// and far plane words there when its rays' direction signs agree.
+// and far plane words there when its rays' direction signs agree. An empty
+// slot's box spans +inf .. -inf on every axis.
 struct alignas(16) BvhNode4P {
     float ox, oy, oz, pad;
     float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];

@@ -310,7 +310,9 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
                 // on the wave's own rows of the stack array. The counting
                 // form keeps the per-ray walk, whose fetches the byte model
                 // counts.
-                lds_u32* wstack = (lds_u32*)lds_stack + (threadIdx.x / 64) * STACK_MAX;
+                static_assert(TRAVERSAL_BLOCK / 64 * PACKET_STACK_ROW <= STACK_LDS * TRAVERSAL_BLOCK,
+                              "the packet stack rows fit the block's LDS stack");
+                lds_u32* wstack = (lds_u32*)lds_stack + (threadIdx.x / 64) * PACKET_STACK_ROW;
                 h = trace_packet(s, o, d, mask, key.x, key.y, wstack, ray_nodes, ray_tris);
             } else {
                 h = trace_closest<W8>(s, o, d, 1e30f, mask, st, key.x, key.y, bounce, ray_nodes, ray_tris);

@@ -483,13 +483,16 @@ __device__ __forceinline__ TraceHit trace_closest(const SceneView& s, v3 o, v3 d
 // nodes (BvhNode4P): the quantized node's boxes with every plane as an f32
 // offset, met with one fma per plane — the per-lane walk's test rounded
 // once less, conservative within the same outward padding. `wstack` holds
-// STACK_MAX entries for this wave.
+// PACKET_STACK_ROW entries for this wave.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // SIGNS: bit k set when every lane's direction is negative on axis k (its
 // near plane on that axis is the high one), or PACKET_MIXED_SIGNS when the
 // lanes differ on some axis and each orders every plane pair itself.
 constexpr int PACKET_MIXED_SIGNS = 8;
+// Entries of a wave's packet stack row: STACK_MAX and the spare entries a
+// block push may write above the top.
+constexpr uint32_t PACKET_STACK_ROW = STACK_MAX + 4;
 template <int SIGNS>
 __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const SlabRay& r, const WRay& w,
                                                        uint32_t mask, uint32_t pixel, uint32_t sample,
@@ -534,8 +537,13 @@ __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const
             }
             const float t0 = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
             const float t1 = fminf(fminf(bx, by), fminf(bz, best.t)) * 1.0000004f;
-            const bool hit = t0 <= t1 && n.child[c] != CHILD_EMPTY;
-            entered[c] = __ballot(hit);
+            // the lanes whose ray enters the child (ordered t0 <= t1: a NaN
+            // never enters), taken straight from the compare; an empty slot's
+            // inverted box is never entered when the near planes are picked
+            // by sign, and is masked out when each lane orders its planes
+            const bool hit = t0 <= t1;
+            const uint64_t in = __builtin_amdgcn_fcmpf(t0, t1, 5 /* ordered <= */);
+            entered[c] = !MIXED || n.child[c] != CHILD_EMPTY ? in : 0;
             key[c] = hit ? __float_as_uint(t0) : 0x7f800000u;  // t0 >= 0: its bits order as the values
         }
         const uint64_t any = entered[0] | entered[1] | entered[2] | entered[3];
@@ -551,7 +559,8 @@ __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const bool used = entered[c] != 0;
-            k[c] = used ? wave_uniform(__builtin_amdgcn_readlane(key[c], lead)) : 0xffffffffu;
+            const uint32_t lead_key = __builtin_amdgcn_readlane(key[c], lead);
+            k[c] = used ? lead_key : 0xffffffffu;
             ref[c] = used ? n.child[c] : NODE_DONE;
             nh += used ? 1u : 0u;
         }
@@ -570,13 +579,20 @@ __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const
         CN_UCE(1, 3);
         CN_UCE(1, 2);
 #undef CN_UCE
-        // push the rest farthest first (a push past STACK_MAX is dropped, as
-        // the per-lane stack does; the builder's bound keeps it from happening)
-        for (uint32_t c = nh; c-- > 1;) {
-            if (sp < STACK_MAX) {
-                if (__lane_id() == leader) wstack[sp] = ref[c];
-                ++sp;
+        // Push the rest farthest first, as one block: entries sp .. sp + nh - 2
+        // get ref[nh - 1] .. ref[1] (the block's other entries lie above the
+        // new top, in the row's spare entries at worst). Pushes past
+        // STACK_MAX are dropped, as the per-lane stack drops them; the
+        // builder's bound keeps it from happening.
+        if (nh > 1) {
+            const uint32_t p0 = nh == 4 ? ref[3] : nh == 3 ? ref[2] : ref[1];
+            const uint32_t p1 = nh == 4 ? ref[2] : ref[1];
+            if (__lane_id() == leader) {
+                wstack[sp] = p0;
+                wstack[sp + 1] = p1;
+                wstack[sp + 2] = ref[1];
             }
+            sp = min(sp + nh - 1, STACK_MAX);
         }
         cur = wave_uniform(ref[0]);
     }
