@@ -410,9 +410,31 @@ struct TrisPerLane {
     const BvhTriangle* tris;
     __device__ __forceinline__ BvhTriangle operator[](uint32_t i) const { return tris[i]; }
 };
+// A triangle (48 bytes at a wave-uniform index) in one scalar round trip,
+// waited for in the same block as the node fetch is.
+typedef uint32_t sgpr_x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t sgpr_x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ BvhTriangle load_packet_triangle(const BvhTriangle* tris, uint32_t index) {
+    const uint64_t at = reinterpret_cast<uint64_t>(tris + index);
+    sgpr_x8 a;
+    sgpr_x4 b;
+    asm volatile(
+        "s_load_dwordx8 %0, %2, 0x0\n\t"
+        "s_load_dwordx4 %1, %2, 0x20\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(a), "=&s"(b)
+        : "s"(at));
+    BvhTriangle t;
+    uint32_t* w = reinterpret_cast<uint32_t*>(&t);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = a[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[8 + k] = b[k];
+    return t;
+}
 struct TrisUniform {
     const BvhTriangle* tris;
-    __device__ __forceinline__ BvhTriangle operator[](uint32_t i) const { return load_uniform(tris, i); }
+    __device__ __forceinline__ BvhTriangle operator[](uint32_t i) const { return load_packet_triangle(tris, i); }
 };
 
 struct TraceHit {
