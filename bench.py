@@ -273,8 +273,9 @@ def measure_plain(ctx, name, steps, warmup, capacity=0, data=None, cpu=False, bv
     times = renderer.kernel_times()
     renderer.set_profiling(False)
 
-    # the ranges' counter lines: 4 queues x 8 sub-queues per bounce, plus a spare bounce
-    lines = (depth + 1) * 4 * 8 * waves_per_frame(width, height, spp, renderer.capacity)
+    # the ranges' counter lines: 4 queues x 8 sub-queues per bounce, plus a spare
+    # bounce, then the tail launch's 8 item counters
+    lines = ((depth + 1) * 4 * 8 + 8) * waves_per_frame(width, height, spp, renderer.capacity)
     frame_bytes, totals = algorithmic_bytes(counts, by_bounce, info, pixels * spp, aov, walked, tail, lines)
     value = pixels * spp * steps / elapsed / 1e6
     grays = (totals["rays"] + totals["shadows"]) * steps / elapsed / 1e9

@@ -272,7 +272,8 @@ int enqueue_wave(cn_renderer* r, std::vector<HostJob> jobs, bool aov, const Wave
                 const bool tail = b == tail_bounce;
                 CN_HIP(launch_bounce(r->stream, r->shape, jobs[rg.g0].scene->view, pool, tg, counters, dev + rg.g0,
                                      uint32_t(rg.g1 - rg.g0), rg.base, rg.count, b, r->light_sampling, r->counting,
-                                     tail, r->profiler.get()));
+                                     tail, counters + tail_claim_index(r->max_bounces), r->ctx->compute_units,
+                                     r->profiler.get()));
                 if (tail) break;
             }
         }

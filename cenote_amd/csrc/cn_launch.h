@@ -71,13 +71,15 @@ struct Profiler {
 hipError_t launch_raygen(hipStream_t stream, uint32_t* counters, uint32_t counter_lines, Profiler* prof);
 
 // One bounce: trace, then shade_miss and shade_surface — or, with `tail`,
-// trace and then the tail launch that finishes every live path. Bounce 0's
-// trace generates the camera paths of slots [base, base + range_count) of a
-// wave whose jobs (sorted by slot base) are `jobs[0 .. job_count)`.
+// trace and then the tail launch that finishes every live path (its item
+// counters at `tail_claims`, zeroed with the range's other counters; its
+// grid the waves `compute_units` CUs hold at once). Bounce 0's trace
+// generates the camera paths of slots [base, base + range_count) of a wave
+// whose jobs (sorted by slot base) are `jobs[0 .. job_count)`.
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, const JobRecord* jobs, uint32_t job_count,
                          uint32_t base, uint32_t range_count, uint32_t bounce, uint32_t light_sampling, int counting,
-                         bool tail, Profiler* prof);
+                         bool tail, uint32_t* tail_claims, uint32_t compute_units, Profiler* prof);
 
 // Fold every job's slots into its film sums (jobs sorted by pixel base;
 // total_pixels is the sum of the jobs' view pixels).

@@ -35,9 +35,14 @@ constexpr uint32_t COUNTER_STRIDE = 32;   // u32 words per counter: one 128-byte
 // Most pixel ranges a single wave may walk (pixels / capacity, rounded up).
 constexpr uint32_t MAX_RANGES = 64;
 // Counters one range holds: Q_COUNT x QUEUE_SPLIT per bounce, plus one
-// spare bounce.
+// spare bounce, then QUEUE_SPLIT striped item counters for the tail launch.
 constexpr uint32_t counter_lines_per_range(uint32_t max_bounces) {
-    return (max_bounces + 1) * Q_COUNT * QUEUE_SPLIT;
+    return (max_bounces + 1) * Q_COUNT * QUEUE_SPLIT + QUEUE_SPLIT;
+}
+// The first of the tail launch's striped item counters (the others follow,
+// COUNTER_STRIDE words apart).
+constexpr uint32_t tail_claim_index(uint32_t max_bounces) {
+    return (max_bounces + 1) * Q_COUNT * QUEUE_SPLIT * COUNTER_STRIDE;
 }
 constexpr uint32_t counters_per_range(uint32_t max_bounces) {
     return counter_lines_per_range(max_bounces) * COUNTER_STRIDE;

@@ -789,7 +789,7 @@ __global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // after both walks and before it shades the next hit.
 template <bool W8>
 __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu(TAIL_WAVES))) void k_tail(
-    SceneView s, PathPool p, WaveTargets tg, uint32_t* counters, uint32_t base, uint32_t bounce,
+    SceneView s, PathPool p, WaveTargets tg, uint32_t* counters, uint32_t* claims, uint32_t base, uint32_t bounce,
     uint32_t light_sampling) {
     __shared__ uint32_t lds_stack[STACK_LDS * TRAVERSAL_BLOCK];
     const uint32_t n_miss = queue_total(counters, bounce, Q_MISS);
@@ -799,78 +799,123 @@ __global__ __launch_bounds__(TRAVERSAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     uint32_t nodes = 0, tris = 0;  // the walks' fetch counts, not kept here
     const bool helper = threadIdx.x & 1u;
     const uint32_t owner = __lane_id() & ~1u, mate = __lane_id() | 1u;
-    for (uint32_t first = blockIdx.x * (TRAVERSAL_BLOCK / 2); first < n; first += lanes / 2) {
-        const uint32_t i = first + threadIdx.x / 2;  // one item per pair
-        if (i >= n) continue;
-        if (i < n_miss) {
-            if (!helper)
-                miss_path(s, p, tg, p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)], base,
-                          bounce, light_sampling);
+    // Items: the miss-queue entries [0, n_miss), then the hit-queue entries
+    // [n_miss, n), one per lane pair. Pairs take items as they free up: a
+    // wave claims one item for each of its idle pairs with one atomic on one
+    // of QUEUE_SPLIT striped counters (stripe k deals items k, k + QUEUE_SPLIT,
+    // ...), moving to a stripe with items left once its own runs out, so no
+    // pair waits on the longest path of its wave while items remain. A pair
+    // carries its path from bounce to bounce itself, in order.
+    uint32_t stripe = (blockIdx.x * (TRAVERSAL_BLOCK / 64) + threadIdx.x / 64) % QUEUE_SPLIT;  // wave-uniform
+    bool items_left = true;  // wave-uniform
+    bool live = false;  // the pair carries a path (the same in both lanes)
+    uint32_t path = 0, b = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!live && !helper);  // the idle pairs' owners
+        if (idle != 0 && items_left) {
+            const uint32_t want = __popcll(idle);
+            const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t c = 0;
+            if (__lane_id() == leader) c = atomicAdd(&claims[stripe * COUNTER_STRIDE], want);
+            c = __builtin_amdgcn_readlane(c, leader);
+            if (!live) {
+                const uint32_t rank = __popcll(idle & ((1ull << owner) - 1));
+                const uint32_t i = stripe + QUEUE_SPLIT * (c + rank);
+                if (i < n_miss) {
+                    const uint32_t entry = p.miss_queue[queue_slot(counters, bounce, Q_MISS, i, p.sub_capacity)];
+                    if (!helper) miss_path(s, p, tg, entry, base, bounce, light_sampling);
+                } else if (i < n) {
+                    path = p.hit_queue[queue_slot(counters, bounce, Q_HIT, i - n_miss, p.sub_capacity)] &
+                           HIT_PATH_MASK;
+                    b = bounce;
+                    live = true;
+                }
+            }
+            if (stripe + QUEUE_SPLIT * (c + want) >= n) {
+                // This stripe has no items left: read every stripe's counter
+                // at once and move to one that has (counters only grow, so
+                // when none seems to, none has).
+                const uint32_t k = __lane_id();
+                uint32_t taken = 0xffffffffu;
+                if (k < QUEUE_SPLIT)
+                    taken = __hip_atomic_load(&claims[k * COUNTER_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t open = __ballot(k < QUEUE_SPLIT && k + QUEUE_SPLIT * taken < n);
+                if (open == 0) items_left = false;
+                else stripe = __ffsll((unsigned long long)open) - 1;
+            }
+        }
+        if (__ballot(live) == 0) {
+            if (!items_left) break;
             continue;
         }
-        const uint32_t path =
-            p.hit_queue[queue_slot(counters, bounce, Q_HIT, i - n_miss, p.sub_capacity)] & HIT_PATH_MASK;
-        for (uint32_t b = bounce;; ++b) {
-            bool connect = false, next = false, below = false;
-            ShadowRecord sh{};
-            if (!helper)
-                next = shade_path(s, p, tg, s.tables, path, base, b, light_sampling,
-                                  [&](bool push, const ShadowRecord& r) {
-                                      connect = push;
-                                      sh = r;
-                                  },
-                                  below);
-            // The owner hands its decisions and the connection to its mate.
-            connect = __shfl(connect ? 1 : 0, owner) != 0;
-            next = __shfl(next ? 1 : 0, owner) != 0;
-            sh.origin = V3(__shfl(sh.origin.x, owner), __shfl(sh.origin.y, owner), __shfl(sh.origin.z, owner));
-            sh.direction =
-                V3(__shfl(sh.direction.x, owner), __shfl(sh.direction.y, owner), __shfl(sh.direction.z, owner));
-            sh.tmax = __shfl(sh.tmax, owner);
-            sh.instance = __shfl(sh.instance, owner);
-            sh.primitive = __shfl(sh.primitive, owner);
-            // The owner walks the next ray, the mate the connection.
-            v3 o = sh.origin, d = sh.direction;
-            float tmax = sh.tmax;
-            uint2 key = make_uint2(0u, 0u);
-            if (!helper && next) {
-                o = xyz(p.origin[path]);
-                d = xyz(p.direction[path]);
-                key = p.key[path];
+        if (live) {  // one bounce of each pair's path
+                bool connect = false, next = false, below = false;
+                ShadowRecord sh{};
+                if (!helper)
+                    next = shade_path(s, p, tg, s.tables, path, base, b, light_sampling,
+                                      [&](bool push, const ShadowRecord& r) {
+                                          connect = push;
+                                          sh = r;
+                                      },
+                                      below);
+                // The owner hands its decisions and the connection to its mate.
+                connect = __shfl(connect ? 1 : 0, owner) != 0;
+                next = __shfl(next ? 1 : 0, owner) != 0;
+                sh.origin = V3(__shfl(sh.origin.x, owner), __shfl(sh.origin.y, owner), __shfl(sh.origin.z, owner));
+                sh.direction =
+                    V3(__shfl(sh.direction.x, owner), __shfl(sh.direction.y, owner), __shfl(sh.direction.z, owner));
+                sh.tmax = __shfl(sh.tmax, owner);
+                sh.instance = __shfl(sh.instance, owner);
+                sh.primitive = __shfl(sh.primitive, owner);
+                // The owner walks the next ray, the mate the connection.
+                v3 o = sh.origin, d = sh.direction;
+                float tmax = sh.tmax;
+                uint2 key = make_uint2(0u, 0u);
+                if (!helper && next) {
+                    o = xyz(p.origin[path]);
+                    d = xyz(p.direction[path]);
+                    key = p.key[path];
+                }
+                if (!helper) tmax = 1e30f;
+                const SlabRay r = make_slab(o, d);
+                const WRay w = make_wray(o, d);
+                TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
+                bool occluded = false, translucent = false;
+                // As the trace kernel takes a connection: a solid hit short of a
+                // triangle light occludes without the closest hit.
+                const float t_early = sh.instance == LIGHT_NONE ? 3.0e38f : sh.tmax * 0.998f;
+                if (helper ? connect : next) {
+                    walk(tree_of<W8>(s), r, best.t, st, nodes, [&](uint32_t ref) {
+                        if (helper) {
+                            occluded = visibility_leaf(s, w, ref, sh.instance, sh.primitive, t_early, best, translucent,
+                                                       tris);
+                            return occluded;
+                        }
+                        closest_leaf(s, TrisPerLane{s.tris}, w, ref, MASK_ALL, key.x, key.y, b + 1, best, tris);
+                        return false;
+                    });
+                }
+                float att = 0.0f;
+                if (helper && connect && !occluded && !(best.hit && !(best.inst == sh.instance && best.prim == sh.primitive)))
+                    att = translucent ? shadow_attenuation<W8>(s, r, w, best.hit ? best.t : sh.tmax, sh.instance,
+                                                               sh.primitive, st, nodes)
+                                      : 1.0f;
+                att = __shfl(att, mate);
+                if (!helper && connect && att > 0.0f) add_pixel(tg.radiance, sh.pixel, sh.contrib * att, 0.0f);
+            if (!next) {
+                live = false;
+            } else {
+                const bool hit = __shfl(best.hit ? 1 : 0, owner) != 0;
+                if (!hit) {
+                    if (!helper) miss_path(s, p, tg, path, base, b + 1, light_sampling);
+                    live = false;
+                } else {
+                    if (!helper)
+                        p.hit[path] =
+                            make_uint4(best.inst, best.prim, __float_as_uint(best.b1), __float_as_uint(best.b2));
+                    ++b;
+                }
             }
-            if (!helper) tmax = 1e30f;
-            const SlabRay r = make_slab(o, d);
-            const WRay w = make_wray(o, d);
-            TraceHit best{false, tmax, 0u, 0u, 0.0f, 0.0f, 0u};
-            bool occluded = false, translucent = false;
-            // As the trace kernel takes a connection: a solid hit short of a
-            // triangle light occludes without the closest hit.
-            const float t_early = sh.instance == LIGHT_NONE ? 3.0e38f : sh.tmax * 0.998f;
-            if (helper ? connect : next) {
-                walk(tree_of<W8>(s), r, best.t, st, nodes, [&](uint32_t ref) {
-                    if (helper) {
-                        occluded = visibility_leaf(s, w, ref, sh.instance, sh.primitive, t_early, best, translucent,
-                                                   tris);
-                        return occluded;
-                    }
-                    closest_leaf(s, TrisPerLane{s.tris}, w, ref, MASK_ALL, key.x, key.y, b + 1, best, tris);
-                    return false;
-                });
-            }
-            float att = 0.0f;
-            if (helper && connect && !occluded && !(best.hit && !(best.inst == sh.instance && best.prim == sh.primitive)))
-                att = translucent ? shadow_attenuation<W8>(s, r, w, best.hit ? best.t : sh.tmax, sh.instance,
-                                                           sh.primitive, st, nodes)
-                                  : 1.0f;
-            att = __shfl(att, mate);
-            if (!helper && connect && att > 0.0f) add_pixel(tg.radiance, sh.pixel, sh.contrib * att, 0.0f);
-            if (!next) break;
-            const bool hit = __shfl(best.hit ? 1 : 0, owner) != 0;
-            if (!hit) {
-                if (!helper) miss_path(s, p, tg, path, base, b + 1, light_sampling);
-                break;
-            }
-            if (!helper) p.hit[path] = make_uint4(best.inst, best.prim, __float_as_uint(best.b1), __float_as_uint(best.b2));
         }
     }
 }
@@ -1177,7 +1222,7 @@ hipError_t launch_raygen(hipStream_t stream, uint32_t* counters, uint32_t counte
 hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const SceneView& s, const PathPool& p,
                          const WaveTargets& tg, uint32_t* counters, const JobRecord* jobs, uint32_t job_count,
                          uint32_t base, uint32_t range_count, uint32_t bounce, uint32_t light_sampling, int counting,
-                         bool tail, Profiler* prof) {
+                         bool tail, uint32_t* tail_claims, uint32_t compute_units, Profiler* prof) {
     const uint32_t trace_grid = shape.trace_blocks;
     const uint32_t light_grid = grid_for(range_count, shape.light_blocks);
     // Bounce b's rays and bounce b-1's shadow rays share one launch; the
@@ -1201,16 +1246,19 @@ hipError_t launch_bounce(hipStream_t stream, const LaunchShape& shape, const Sce
 #undef CN_TRACE
     if (prof) prof->end(K_TRACE);
     if (tail) {
-        // shade_surface's block cap in 256-lane blocks, at most the trace
-        // grid (whose lanes the traversal stack's spill area is sized for).
-        const uint32_t tail_grid = shape.shade_blocks < trace_grid ? shape.shade_blocks : trace_grid;
+        // The waves the chip holds at once (TAIL_WAVES per SIMD: one
+        // 256-lane block per CU for each), at most the trace grid (whose
+        // lanes the traversal stack's spill area is sized for): the pairs
+        // claim items until none are left.
+        const uint32_t resident = (compute_units ? compute_units : 1u) * TAIL_WAVES;
+        const uint32_t tail_grid = resident < trace_grid ? resident : trace_grid;
         if (prof) prof->begin(K_TAIL);
         if (wide)
             hipLaunchKernelGGL(k_tail<true>, dim3(tail_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters,
-                               base, bounce, light_sampling);
+                               tail_claims, base, bounce, light_sampling);
         else
             hipLaunchKernelGGL(k_tail<false>, dim3(tail_grid), dim3(TRAVERSAL_BLOCK), 0, stream, s, p, tg, counters,
-                               base, bounce, light_sampling);
+                               tail_claims, base, bounce, light_sampling);
         if (prof) prof->end(K_TAIL);
         return hipGetLastError();
     }
