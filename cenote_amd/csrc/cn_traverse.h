@@ -573,6 +573,13 @@ __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const
             cur = sp == 0 ? NODE_DONE : wave_uniform(wstack[--sp]);
             continue;
         }
+        // One child entered (the common case for a coherent packet): visit
+        // it, with nothing to order or push.
+        const uint32_t entered_count = (entered[0] != 0) + (entered[1] != 0) + (entered[2] != 0) + (entered[3] != 0);
+        if (entered_count == 1) {
+            cur = entered[0] ? n.child[0] : entered[1] ? n.child[1] : entered[2] ? n.child[2] : n.child[3];
+            continue;
+        }
         // The children some lane enters, keyed by the first such lane's
         // entry distances (those it does not enter after, absent ones last).
         const uint32_t lead = __ffsll((unsigned long long)any) - 1;
