@@ -593,6 +593,22 @@ __device__ __forceinline__ TraceHit trace_packet_signs(const SceneView& s, const
             ref[c] = used ? n.child[c] : NODE_DONE;
             nh += used ? 1u : 0u;
         }
+        if (nh == 2) {
+            // Two children entered: one compare orders them (an unused slot
+            // has key ~0, an entered one never does), the farther one is
+            // pushed alone.
+            const uint32_t ka = k[0] != ~0u ? k[0] : k[1] != ~0u ? k[1] : k[2];
+            const uint32_t ra = k[0] != ~0u ? ref[0] : k[1] != ~0u ? ref[1] : ref[2];
+            const uint32_t kb = k[3] != ~0u ? k[3] : k[2] != ~0u ? k[2] : k[1];
+            const uint32_t rb = k[3] != ~0u ? ref[3] : k[2] != ~0u ? ref[2] : ref[1];
+            const bool a_first = ka <= kb;
+            if (sp < STACK_MAX) {
+                if (__lane_id() == leader) wstack[sp] = a_first ? rb : ra;
+                ++sp;
+            }
+            cur = wave_uniform(a_first ? ra : rb);
+            continue;
+        }
 #define CN_UCE(a, b)                                          \
     do {                                                      \
         const bool swap = k[a] > k[b];                        \
