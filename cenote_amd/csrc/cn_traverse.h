@@ -368,22 +368,9 @@ __device__ __forceinline__ void leaf_range(uint32_t ref, uint32_t* first, uint32
     *count = ((ref >> 27) & 15u) + 1u;
 }
 
-// The tree and triangles read as constant memory (address space 4): a load
-// at a wave-uniform address then goes through the scalar unit, once per
-// wave. Records are copied word by word (the scalar loads merge them).
-typedef const __attribute__((address_space(4))) uint32_t const_u32;
-template <typename T>
-__device__ __forceinline__ T load_uniform(const T* array, uint32_t i) {
-    static_assert(sizeof(T) % 4 == 0, "whole words");
-    const_u32* src = (const_u32*)(array + i);
-    T v;
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
-#pragma unroll
-    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
-    return v;
-}
-// A packet node (128 bytes at a wave-uniform address) in one round trip of
-// the scalar unit: two 64-byte loads from one base, waited for in the same
+// The packet walk reads the tree and triangles at wave-uniform addresses
+// through the scalar unit, once per wave. A packet node (128 bytes) comes in
+// one round trip: two 64-byte loads from one base, waited for in the same
 // block, so no load is in flight when the registers are handed back.
 typedef uint32_t sgpr_x16 __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ BvhNode4P load_packet_node(const BvhNode4P* nodes, uint32_t index) {
